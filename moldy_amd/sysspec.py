@@ -1,0 +1,199 @@
+"""Readers for Moldy's control and system-specification files.
+
+Follows the file grammar of read_control (src/input.c:528-573, keyword table
+src/startup.c:143-202), read_sysdef (src/input.c:128-365) and lattice_start
+(src/input.c:375-491), and converts to program units with unit_scale
+(src/convert.c:63-124).  The result is a MoldySystem ready for the force path.
+"""
+from __future__ import annotations
+
+import math
+import re
+
+import numpy as np
+
+from . import units
+from .system import Control, MoldySystem, Species, lattice_h, NPOTP
+
+CONTROL_DEFAULTS = {
+    "cutoff": "0.0", "subcell": "0.0", "density": "1.0", "alpha": "0.0",
+    "k-cutoff": "0.0", "strict-cutoff": "0", "molecular-cutoff": "0",
+    "surface-dipole": "0", "lattice-start": "0", "step": "0.005",
+    "ewald-accuracy": "1.013e-5", "mass-unit": repr(units.AMU),
+    "length-unit": "1.0e-10", "time-unit": "1.0e-13", "charge-unit": repr(units.ELCHG),
+}
+
+
+def _lines(text):
+    """get_line(): skip blank and '#' lines, strip trailing space."""
+    for raw in text.splitlines():
+        s = raw.rstrip()
+        if not s or s.lstrip().startswith("#") and s.startswith("#"):
+            continue
+        yield s
+
+
+def parse_control(text):
+    ctl = dict(CONTROL_DEFAULTS)
+    for line in text.splitlines():
+        line = line.split("#", 1)[0].strip()
+        if not line:
+            continue
+        if line.lower() == "end":
+            break
+        if "=" not in line:
+            continue
+        k, v = line.split("=", 1)
+        ctl[k.strip().lower()] = v.strip()
+    return ctl
+
+
+def parse_sysspec(text):
+    """Returns dict(species=[...], sites={id: (mass, charge, name)}, ptype,
+    pots=[(i, j, [p...])], lattice=None|dict)."""
+    it = iter(list(_lines(text)))
+    species, sites = [], {}
+    line = next(it)
+    while line.split() and line.split()[0].lower() != "end":
+        tok = line.split()
+        name, nmols = tok[0], int(tok[1])
+        fw = len(tok) > 2 and tok[2].lower() == "framework"
+        sp = dict(name=name, nmols=nmols, framework=fw, ids=[], xyz=[])
+        line = next(it)
+        while re.match(r"^\s*[+-]?\d+(\s|$)", line):
+            tok = line.split()
+            sid = int(tok[0])
+            sp["ids"].append(sid)
+            sp["xyz"].append([float(t) for t in tok[1:4]])
+            m = sites.setdefault(sid, {})
+            if len(tok) > 4:
+                m["mass"] = float(tok[4])
+            if len(tok) > 5:
+                m["charge"] = float(tok[5])
+            if len(tok) > 6:
+                m["name"] = tok[6]
+            line = next(it)
+        species.append(sp)
+    ptype_name = next(it).split()[0].lower()
+    ptype = units.POT_NAMES.index(ptype_name)
+    pots = []
+    for line in it:
+        tok = line.split("#", 1)[0].split()
+        if not tok:
+            continue
+        if tok[0].lower() == "end":
+            break
+        pots.append((int(tok[0]), int(tok[1]), [float(t) for t in tok[2:]]))
+    lattice = None
+    rest = list(it)
+    if rest:
+        tok = rest[0].split()
+        cell = [float(t) for t in tok[:6]] + [int(t) for t in tok[6:9]]
+        mols = []
+        for line in rest[1:]:
+            tok = line.split()
+            if tok[0].lower() == "end":
+                break
+            mols.append((tok[0], [float(t) for t in tok[1:]]))
+        lattice = dict(cell=cell, mols=mols)
+    return dict(species=species, sites=sites, ptype=ptype, pots=pots, lattice=lattice)
+
+
+def input_units(ctl):
+    return (float(ctl["mass-unit"]), float(ctl["length-unit"]),
+            float(ctl["time-unit"]), float(ctl["charge-unit"]))
+
+
+def build_system(ctl, spec, name="moldy"):
+    """MoldySystem in program units from parsed control + sys-spec."""
+    uin = input_units(ctl)
+    max_id = max(spec["sites"]) + 1
+    mscale = units.unit_scale((1, 0, 0, 0), uin, units.PROGRAM_UNITS)
+    qscale = units.unit_scale((0, 0, 0, 1), uin, units.PROGRAM_UNITS)
+    mass = np.zeros(max_id)
+    chg = np.zeros(max_id)
+    for sid, m in spec["sites"].items():
+        mass[sid] = m.get("mass", 0.0) * mscale
+        chg[sid] = m.get("charge", 0.0) * qscale
+    ptype = spec["ptype"]
+    pp = np.zeros((max_id, max_id, NPOTP))
+    scale = [units.unit_scale(d, uin, units.PROGRAM_UNITS) for d in units.POT_DIM[ptype]]
+    for i, j, p in spec["pots"]:
+        v = np.zeros(NPOTP)
+        v[:len(p)] = p[:NPOTP]
+        v[:len(scale)] *= scale
+        pp[i, j] = v
+        pp[j, i] = v
+    # species, frameworks last (sort_species keeps relative order otherwise)
+    sp_list = [s for s in spec["species"] if not s["framework"]] + \
+              [s for s in spec["species"] if s["framework"]]
+    species = []
+    for s in sp_list:
+        xyz = np.array(s["xyz"], np.float64)
+        m = mass[np.array(s["ids"])] / mscale if mscale else mass[np.array(s["ids"])]
+        tot = m.sum()
+        com = (m[:, None] * xyz).sum(0) / tot if tot > 0 else xyz.mean(0)
+        rel = xyz - com
+        ns = len(s["ids"])
+        rdof = 0 if ns == 1 else 3
+        species.append(Species(s["name"], s["nmols"], list(s["ids"]), rel, s["framework"], rdof))
+    c = Control(cutoff=float(ctl["cutoff"]), subcell=float(ctl["subcell"]),
+                alpha=float(ctl["alpha"]), k_cutoff=float(ctl["k-cutoff"]),
+                strict_cutoff=bool(int(ctl["strict-cutoff"])),
+                molpbc=bool(int(ctl["molecular-cutoff"])),
+                surface_dipole=bool(int(ctl["surface-dipole"])))
+    nmols = sum(s.nmols for s in species)
+    nmols_r = sum(s.nmols for s in species if s.rdof > 0)
+    if int(ctl["lattice-start"]) and spec["lattice"]:
+        lat = spec["lattice"]
+        a, b, cc, al, be, ga, nx, ny, nz = lat["cell"]
+        h = lattice_h(a, b, cc, al, be, ga, nx, ny, nz)
+        byname = {s.name.lower(): i for i, s in enumerate(species)}
+        cms = [[] for _ in species]
+        qs = [[] for _ in species]
+        # lattice_start builds a reversed linked list of the lines
+        for nm, vals in reversed(lat["mols"]):
+            i = byname[nm.lower()]
+            for ix in range(nx):
+                for iy in range(ny):
+                    for iz in range(nz):
+                        cms[i].append([(vals[0] + ix) / nx - 0.5, (vals[1] + iy) / ny - 0.5,
+                                       (vals[2] + iz) / nz - 0.5])
+                        if species[i].rdof:
+                            qs[i].append(vals[3:7])
+        c_of_m = np.array([x for l in cms for x in l])
+        quat = np.array([x for l in qs for x in l]) if nmols_r else np.zeros((0, 4))
+    else:
+        total_mass = sum(mass[np.array(s.site_ids)].sum() / mscale * s.nmols
+                         for s in species) * mscale
+        L = (total_mass / units.density_to_program(float(ctl["density"]))) ** (1.0 / 3.0)
+        h = np.eye(3) * L
+        c_of_m = np.zeros((nmols, 3))
+        quat = np.tile([1.0, 0, 0, 0], (nmols_r, 1))
+    return MoldySystem(species, mass, chg, pp, ptype, c, h, c_of_m, quat, name)
+
+
+def skew_box(ctl, system):
+    """Box edge of skew_start (src/startup.c:304-327): cube root of mass/density."""
+    mass = sum(system.site_mass[np.asarray(s.site_ids)].sum() * s.nmols for s in system.species)
+    return math.pow(mass / units.density_to_program(float(ctl["density"])), 1.0 / 3.0)
+
+
+def init_cutoffs(alpha, cutoff, k_cutoff, h, nsites, accuracy=1.013e-5):
+    """Automatic Ewald parameters, src/startup.c:713-772."""
+    vol = float(np.linalg.det(h))
+    max_cutoff = min(h[0, 0], h[1, 1], h[2, 2])
+    sqrt_p = math.sqrt(-math.log(accuracy))
+    if alpha >= 0.0:
+        if alpha == 0.0:
+            alpha = math.pow(nsites * units.PI ** 3 / vol ** 2 * 5.5, 1.0 / 6.0)
+            trial = sqrt_p / alpha
+            if cutoff > trial:
+                alpha = sqrt_p / cutoff
+            elif cutoff == 0.0:
+                cutoff = trial
+        elif cutoff == 0.0:
+            cutoff = min(sqrt_p / alpha, max_cutoff)
+        if k_cutoff == 0.0:
+            k_cutoff = 2.0 * alpha * sqrt_p
+    return alpha, cutoff, k_cutoff

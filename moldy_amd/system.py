@@ -1,0 +1,350 @@
+"""System description for the force path, and Moldy-format readers/builders.
+
+``MoldySystem`` holds what Moldy's force_calc()/ewald() see through system_mt,
+spec_mt[], site_mt[], pot_mt[] and the control fields they read
+(src/structs.h:27-161): species ordered with frameworks last
+(src/input.c:95-117), per-id site masses and charges, the symmetric
+potential-parameter table potpar[max_id*max_id][8] (src/input.c:343-349),
+the cell matrix h (upper triangular, columns a, b, c), scaled centre-of-mass
+coordinates and quaternions.  Everything is in program units (see units.py).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+from . import units
+
+NPOTP = 8
+
+
+@dataclasses.dataclass
+class Species:
+    name: str
+    nmols: int
+    site_ids: List[int]
+    p_f_sites: np.ndarray          # [nsites][3] principal-frame coordinates
+    framework: bool = False
+    rdof: int = 0                  # 0 monatomic, 2 linear, 3 general
+
+    @property
+    def nsites(self):
+        return len(self.site_ids)
+
+
+@dataclasses.dataclass
+class Control:
+    cutoff: float = 0.0
+    subcell: float = 0.0
+    alpha: float = -1.0            # <= 0: no Ewald sum
+    k_cutoff: float = 0.0
+    strict_cutoff: bool = False
+    molpbc: bool = False
+    surface_dipole: bool = False
+    step: float = 0.0              # program units (ps)
+
+
+@dataclasses.dataclass
+class MoldySystem:
+    species: List[Species]
+    site_mass: np.ndarray           # [max_id]
+    site_charge: np.ndarray         # [max_id] program units
+    potpar: np.ndarray              # [max_id, max_id, NPOTP] program units
+    ptype: int
+    control: Control
+    h: np.ndarray                   # [3,3]
+    c_of_m: np.ndarray              # [nmols,3] scaled (fractional) coordinates
+    quat: np.ndarray                # [nmols_r,4]
+    name: str = "system"
+
+    # ---------------------------------------------------------------- sizes
+    @property
+    def max_id(self):
+        return len(self.site_charge)
+
+    @property
+    def nspecies(self):
+        return len(self.species)
+
+    @property
+    def nmols(self):
+        return sum(s.nmols for s in self.species)
+
+    @property
+    def nmols_r(self):
+        return sum(s.nmols for s in self.species if s.rdof > 0)
+
+    @property
+    def nsites(self):
+        return sum(s.nmols * s.nsites for s in self.species)
+
+    @property
+    def n_potpar(self):
+        return units.POT_NPAR[self.ptype]
+
+    def site_charges(self):
+        """chg[nsites] as eval_forces builds it (src/accel.c:480-484)."""
+        out = []
+        for s in self.species:
+            q = self.site_charge[np.asarray(s.site_ids)]
+            out.append(np.tile(q, s.nmols))
+        return np.concatenate(out).astype(np.float64)
+
+    def site_id_array(self):
+        return np.concatenate([np.tile(np.asarray(s.site_ids, np.int32), s.nmols)
+                               for s in self.species]).astype(np.int32)
+
+    def mol_index(self):
+        out, m = [], 0
+        for s in self.species:
+            out.append(np.repeat(np.arange(m, m + s.nmols, dtype=np.int32), s.nsites))
+            m += s.nmols
+        return np.concatenate(out)
+
+    def volume(self):
+        return float(np.linalg.det(self.h))
+
+    # ------------------------------------------------------ flat description
+    def flat(self):
+        """Contiguous arrays in the layout the C ABIs take (row-major h)."""
+        sp = self.species
+        return dict(
+            spec_nmols=np.array([s.nmols for s in sp], np.int32),
+            spec_nsites=np.array([s.nsites for s in sp], np.int32),
+            spec_framework=np.array([int(s.framework) for s in sp], np.int32),
+            spec_rdof=np.array([s.rdof for s in sp], np.int32),
+            spec_site_id=np.concatenate([np.asarray(s.site_ids, np.int32) for s in sp]),
+            spec_p_f_sites=np.ascontiguousarray(
+                np.concatenate([np.asarray(s.p_f_sites, np.float64).reshape(-1, 3)
+                                for s in sp])),
+            site_charge=np.ascontiguousarray(self.site_charge, np.float64),
+            potpar=np.ascontiguousarray(self.potpar.reshape(-1, NPOTP), np.float64),
+            h=np.ascontiguousarray(self.h, np.float64),
+            c_of_m=np.ascontiguousarray(self.c_of_m, np.float64),
+            quat=np.ascontiguousarray(self.quat.reshape(-1, 4), np.float64)
+            if self.nmols_r else np.zeros((1, 4)),
+        )
+
+    def copy(self):
+        return dataclasses.replace(
+            self, species=[dataclasses.replace(s, p_f_sites=s.p_f_sites.copy())
+                           for s in self.species],
+            site_mass=self.site_mass.copy(), site_charge=self.site_charge.copy(),
+            potpar=self.potpar.copy(), control=dataclasses.replace(self.control),
+            h=self.h.copy(), c_of_m=self.c_of_m.copy(), quat=self.quat.copy())
+
+
+# ------------------------------------------------------------------ helpers
+
+def principal_frame(sites, masses):
+    """Centre of mass at the origin and axes along the principal moments.
+
+    Moldy does this in initialise_sysdef (src/startup.c:455); the rotation
+    convention only has to be the one the quaternions refer to, so builders
+    that define molecules in a symmetric pose use it directly.
+    """
+    sites = np.asarray(sites, np.float64)
+    m = np.asarray(masses, np.float64)
+    com = (m[:, None] * sites).sum(0) / max(m.sum(), 1e-300)
+    return sites - com, com
+
+
+def lattice_h(a, b, c, alpha, beta, gamma, nx, ny, nz):
+    """Cell matrix from lattice constants, src/input.c:404-414."""
+    dtor = units.PI / 180.0
+    ca, cb, cg = math.cos(alpha * dtor), math.cos(beta * dtor), math.cos(gamma * dtor)
+    sg = math.sin(gamma * dtor)
+    h = np.zeros((3, 3))
+    h[0, 0] = nx * a
+    h[0, 1] = ny * b * cg
+    h[1, 1] = ny * b * sg
+    h[0, 2] = nz * c * cb
+    h[1, 2] = nz * c / sg * (ca - cb * cg)
+    h[2, 2] = nz * c / sg * math.sqrt(1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg)
+    return h
+
+
+def random_quaternions(rng, n):
+    """Uniform random unit quaternions (Shoemake), renormalised to 1e-12."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2),
+                  np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3),
+                  np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def empty_potpar(max_id):
+    return np.zeros((max_id, max_id, NPOTP))
+
+
+def set_pair(potpar, i, j, params):
+    potpar[i, j, :len(params)] = params
+    potpar[j, i, :len(params)] = params
+
+
+# --------------------------------------------------------------- builders
+
+TIP4P_GEOM = dict(r_oh=0.9572, hoh_deg=104.52, r_om=0.15)
+M_O, M_H = 15.9994, 1.00794
+
+
+def tip4p_sites():
+    """Lab-frame TIP4P geometry: O at origin, H in the xz plane, M on -z."""
+    half = math.radians(TIP4P_GEOM["hoh_deg"] / 2)
+    hx, hz = TIP4P_GEOM["r_oh"] * math.sin(half), -TIP4P_GEOM["r_oh"] * math.cos(half)
+    sites = np.array([[0, 0, 0], [hx, 0, hz], [-hx, 0, hz], [0, 0, -TIP4P_GEOM["r_om"]]])
+    pf, _ = principal_frame(sites, [M_O, M_H, M_H, 0.0])
+    return pf
+
+
+def water_species(nmols, ids=(1, 2, 2, 3)):
+    return Species("water", nmols, list(ids), tip4p_sites(), False, 3)
+
+
+def _cubic_box(total_mass_amu, rho_g_cm3):
+    return (total_mass_amu / units.density_to_program(rho_g_cm3)) ** (1.0 / 3.0)
+
+
+def water_box(nmols=4096, seed=1, jitter=0.1, rho=0.997, cutoff=9.0, alpha=0.30,
+              accuracy=1e-6, name=None):
+    """Rigid 4-site (TIP4P-geometry) water on a jittered cubic lattice."""
+    rng = _rng(seed)
+    mol_mass = M_O + 2 * M_H
+    L = _cubic_box(nmols * mol_mass, rho)
+    n = int(round(nmols ** (1.0 / 3.0)))
+    while n ** 3 < nmols:
+        n += 1
+    grid = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"),
+                    -1).reshape(-1, 3)[:nmols]
+    pos = (grid + 0.5) / n - 0.5 + rng.normal(0, jitter / L, (nmols, 3))
+    pos -= np.floor(pos + 0.5)
+    max_id = 4
+    q = np.zeros(max_id)
+    q[2], q[3] = units.charge_to_program(0.52), units.charge_to_program(-1.04)
+    mass = np.array([0, M_O, M_H, 0.0])
+    pp = empty_potpar(max_id)
+    set_pair(pp, 1, 1, units.potpar_to_program(0, [4 * 0.6485, 3.154]))
+    ctl = Control(cutoff=cutoff, alpha=alpha,
+                  k_cutoff=2 * alpha * math.sqrt(-math.log(accuracy)) if alpha > 0 else 0.0,
+                  step=0.001)
+    return MoldySystem([water_species(nmols)], mass, q, pp, 0, ctl, np.eye(3) * L,
+                       pos, random_quaternions(rng, nmols), name or f"water{nmols}")
+
+
+# Ion LJ parameters (sigma A, epsilon kJ/mol): Mg2+ after Aqvist (1990),
+# Cl- after Smith & Dang (1994); cross terms by Lorentz-Berthelot mixing.
+ION_LJ = {"Mg": (1.644, 3.66118), "Cl": (4.40, 0.41840)}
+WATER_LJ = (3.154, 0.6485)
+M_MG, M_CL = 24.305, 35.453
+
+
+def salt_water(nwater=32768, nmg=600, ncl=1200, seed=3, jitter=0.1, rho=1.0,
+               cutoff=10.0, alpha=0.30, accuracy=1e-6, name=None):
+    """Aqueous MgCl2: rigid 4-site water plus Mg2+ and Cl- ions placed at
+    random, non-overlapping sites of a cubic lattice (seeded)."""
+    rng = _rng(seed)
+    nmol = nwater + nmg + ncl
+    mass_tot = nwater * (M_O + 2 * M_H) + nmg * M_MG + ncl * M_CL
+    L = _cubic_box(mass_tot, rho)
+    n = int(math.ceil(nmol ** (1.0 / 3.0)))
+    grid = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"),
+                    -1).reshape(-1, 3)
+    pick = rng.permutation(len(grid))[:nmol]
+    pos = (grid[pick] + 0.5) / n - 0.5 + rng.normal(0, jitter / L, (nmol, 3))
+    pos -= np.floor(pos + 0.5)
+    # species order: water, Mg, Cl (no frameworks)
+    max_id = 6
+    q = np.zeros(max_id)
+    q[2], q[3] = units.charge_to_program(0.52), units.charge_to_program(-1.04)
+    q[4], q[5] = units.charge_to_program(2.0), units.charge_to_program(-1.0)
+    mass = np.array([0, M_O, M_H, 0.0, M_MG, M_CL])
+    lj = {1: WATER_LJ, 4: ION_LJ["Mg"], 5: ION_LJ["Cl"]}
+    pp = empty_potpar(max_id)
+    for i in lj:
+        for j in lj:
+            if j < i:
+                continue
+            s = 0.5 * (lj[i][0] + lj[j][0])
+            e = math.sqrt(lj[i][1] * lj[j][1])
+            set_pair(pp, i, j, units.potpar_to_program(0, [4 * e, s]))
+    ctl = Control(cutoff=cutoff, alpha=alpha,
+                  k_cutoff=2 * alpha * math.sqrt(-math.log(accuracy)), step=0.001)
+    sp = [water_species(nwater),
+          Species("magnesium", nmg, [4], np.zeros((1, 3)), False, 0),
+          Species("chloride", ncl, [5], np.zeros((1, 3)), False, 0)]
+    return MoldySystem(sp, mass, q, pp, 0, ctl, np.eye(3) * L, pos,
+                       random_quaternions(rng, nwater),
+                       name or f"saltwater{nwater}+{nmg}+{ncl}")
+
+
+def argon_fcc(ncell=6, a=5.7266, seed=5, cutoff=10.2, eps_K=119.8, sigma=3.405,
+              jitter=0.0, name=None):
+    """LJ argon on an fcc lattice; p0 = 4 epsilon (Moldy's LJ has no factor 4)."""
+    rng = _rng(seed)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.stack(np.meshgrid(np.arange(ncell), np.arange(ncell), np.arange(ncell),
+                                 indexing="ij"), -1).reshape(-1, 3)
+    pos = ((cells[:, None, :] + basis[None]) / ncell).reshape(-1, 3) - 0.5 + 0.25 / ncell
+    pos += rng.normal(0, jitter / (a * ncell), pos.shape)
+    pos -= np.floor(pos + 0.5)
+    eps_kjmol = eps_K * units.KB_SI * units.AVOGAD / 1000.0
+    max_id = 2
+    pp = empty_potpar(max_id)
+    set_pair(pp, 1, 1, units.potpar_to_program(0, [4 * eps_kjmol, sigma]))
+    ctl = Control(cutoff=cutoff, alpha=-1.0, step=0.01)
+    L = a * ncell
+    sp = [Species("argon", len(pos), [1], np.zeros((1, 3)), False, 0)]
+    return MoldySystem(sp, np.array([0, 39.948]), np.zeros(max_id), pp, 0, ctl,
+                       np.eye(3) * L, pos, np.zeros((0, 4)), name or f"argon{len(pos)}")
+
+
+def silica(ncell=10, seed=7, disp=0.05, cutoff=10.0, alpha=0.30, accuracy=1e-6,
+           a=7.16, name=None):
+    """SiO2 with BKS-type Buckingham parameters (eV) on an ideal
+    beta-cristobalite-type cubic lattice (8 SiO2 per cell, so 10^3 cells give
+    exactly 8000 Si and 16000 O), with seeded Gaussian displacements."""
+    rng = _rng(seed)
+    fcc = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    si = np.concatenate([fcc, fcc + 0.25])
+    # O midway between each bonded Si pair of the diamond lattice (2 per Si)
+    o = []
+    d_off = np.array([[.25, .25, .25], [.25, -.25, -.25], [-.25, .25, -.25], [-.25, -.25, .25]])
+    for s in fcc:
+        for d in d_off:
+            o.append(s + d / 2)
+    o = np.array(o)
+    cells = np.stack(np.meshgrid(np.arange(ncell), np.arange(ncell), np.arange(ncell),
+                                 indexing="ij"), -1).reshape(-1, 3)
+    L = a * ncell
+    sipos = ((cells[:, None] + si[None]) / ncell).reshape(-1, 3)
+    opos = ((cells[:, None] + o[None]) / ncell).reshape(-1, 3)
+    pos = np.concatenate([opos, sipos])
+    pos = pos - 0.5 + rng.normal(0, disp / L, pos.shape)
+    pos -= np.floor(pos + 0.5)
+    ev_units = (units.AMU, 1e-10, 1.0181e-14, units.ELCHG)
+    max_id = 3
+    q = np.zeros(max_id)
+    q[1] = units.charge_to_program(-1.2, ev_units)
+    q[2] = units.charge_to_program(2.4, ev_units)
+    pp = empty_potpar(max_id)
+    set_pair(pp, 1, 1, units.potpar_to_program(1, [175.0000, 1388.7730, 2.76000], ev_units))
+    set_pair(pp, 1, 2, units.potpar_to_program(1, [133.5381, 18003.7572, 4.87318], ev_units))
+    ctl = Control(cutoff=cutoff, alpha=alpha,
+                  k_cutoff=2 * alpha * math.sqrt(-math.log(accuracy)), step=0.001)
+    sp = [Species("oxygen", len(opos), [1], np.zeros((1, 3)), False, 0),
+          Species("silicon", len(sipos), [2], np.zeros((1, 3)), False, 0)]
+    return MoldySystem(sp, np.array([0, 15.9994, 28.0855]), q, pp, 1, ctl,
+                       np.eye(3) * L, pos, np.zeros((0, 4)), name or f"silica{len(pos)}")
+
+
+def scaled_water(nmols, seed=1, **kw):
+    """Small water boxes for tests (same model and density as config 2)."""
+    return water_box(nmols, seed=seed, **kw)
