@@ -1,0 +1,127 @@
+/*
+ * moldy_amd.h -- C ABI of the MI355X force engine (libmoldy_amd.so).
+ *
+ * Plain pointers and sizes only.  All entry points return 0 on success or a
+ * MGPU_E* status; mgpu_last_error() gives the message.  Arrays are FP64 in
+ * Moldy's program units.  Site arrays are structure-of-arrays [3][ld] (the
+ * layout of eval_forces' site/site_force blocks, src/accel.c:422-426); h is
+ * the 3x3 cell matrix, row-major, upper triangular (columns a, b, c).
+ *
+ * The Moldy-signature drop-ins force_calc()/ewald()/poteval() are in
+ * moldy_force.h; they are implemented on top of these calls.
+ */
+#ifndef MOLDY_AMD_H
+#define MOLDY_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mgpu_engine mgpu_engine;
+
+enum {
+   MGPU_OK = 0,
+   MGPU_E_ARG = 1,        /* bad argument                                     */
+   MGPU_E_HIP = 2,        /* HIP runtime error                                */
+   MGPU_E_BIN_RANGE = 3,  /* "Co-ordinate out of range in BIN", force.c:130   */
+   MGPU_E_BIN_ROUND = 4,  /* "Rounding problem in BIN",        force.c:134    */
+   MGPU_E_CUTOFF = 5,     /* CUTOFF fatal,                     force.c:206    */
+   MGPU_E_TONAB = 6,      /* TONAB fatal,                      force.c:562    */
+   MGPU_E_MULTIFW = 7,    /* multiple frameworks,              force.c:1219   */
+   MGPU_E_PTYPE = 8,      /* UNKPTY,                           kernel.c:186   */
+   MGPU_E_STATE = 9,      /* call order (no system set, ...)                  */
+   MGPU_E_CAPACITY = 10   /* a fixed-capacity limit of this engine exceeded   */
+};
+
+enum { MGPU_FP64 = 0, MGPU_MIXED = 1 };
+
+/* System description: the content of system_mt, spec_mt[], site_mt[] charges,
+ * pot_mt[] and the contr_mt fields the path reads (src/structs.h:27-161).
+ * Species are ordered with frameworks last (src/input.c:95-117). */
+typedef struct {
+   int nsites, nmols, nspecies, max_id, ptype, n_potpar;
+   const int    *spec_nmols;      /* [nspecies]                                */
+   const int    *spec_nsites;     /* [nspecies]                                */
+   const int    *spec_framework;  /* [nspecies]                                */
+   const int    *spec_rdof;       /* [nspecies]                                */
+   const int    *spec_site_id;    /* concatenated per-species site ids         */
+   const double *spec_p_f_sites;  /* concatenated principal-frame sites [..][3]*/
+   const double *site_charge;     /* [max_id] charge per site id               */
+   const double *potpar;          /* [max_id*max_id][8], pot_mt.p              */
+   double cutoff, subcell, alpha, k_cutoff;
+   int strict_cutoff, molpbc, surface_dipole;
+} mgpu_system;
+
+/* Diagnostics of one real-space call (the notes/warnings Moldy prints). */
+typedef struct {
+   int nx, ny, nz, n_nabors;      /* subcells; half-shell offsets (Moldy prints 2x) */
+   int n_too_close;               /* TOOCLS pairs, force.c:944                  */
+   int too_close_i[16], too_close_j[16];
+   long n_list_entries;           /* sum of site-neighbour-list lengths         */
+   int max_cell_sites;
+   double eintra;                 /* intramolecular correction, force.c:1167    */
+} mgpu_fc_info;
+
+typedef struct {
+   int hmax, kmax, lmax, nhkl, nsitesxf;
+   double self_energy, sheet_energy;
+} mgpu_ew_info;
+
+/* ---- lifetime ---- */
+int  mgpu_create(mgpu_engine **eng, int device);
+void mgpu_destroy(mgpu_engine *eng);
+const char *mgpu_last_error(const mgpu_engine *eng);
+int  mgpu_set_system(mgpu_engine *eng, const mgpu_system *sys);
+int  mgpu_set_precision(mgpu_engine *eng, int precision);
+
+/* ---- the boundary, host buffers (replaces src/force.c:1108 force_calc) ----
+ * site/site_force: [3][ld]; c_of_m_frac: [nmols][3] (read when molpbc);
+ * chg: [nsites].  Accumulates into site_force, *pe and the upper triangle of
+ * stress (row-major 3x3), as the reference does. */
+int mgpu_force_calc(mgpu_engine *eng, const double h[9], const double *site, int ld,
+                    const double *c_of_m_frac, const double *chg, double *site_force,
+                    double *pe, double stress[9], mgpu_fc_info *info);
+
+/* ---- the boundary, host buffers (replaces src/ewald.c:280 ewald) ---- */
+int mgpu_ewald(mgpu_engine *eng, const double h[9], const double *site, int ld,
+               const double *chg, double *site_force, double *pe, double stress[9],
+               mgpu_ew_info *info);
+
+/* ---- the caller, device resident (restates src/accel.c:398-617) ----
+ * Upload h, scaled centre-of-mass coordinates [nmols][3] and quaternions
+ * [nmols_r][4]; mgpu_eval_forces then runs make_sites, force_calc, ewald and
+ * the glue on the GPU and returns pe[2], dip_mom[3] and the full stress[9].
+ * Molecular forces/torques and site forces stay on the device until
+ * mgpu_download_forces. */
+int mgpu_upload_state(mgpu_engine *eng, const double h[9], const double *c_of_m_frac,
+                      const double *quat);
+int mgpu_eval_forces(mgpu_engine *eng, double pe[2], double dip_mom[3], double stress[9]);
+int mgpu_download_forces(mgpu_engine *eng, double *force, double *torque,
+                         double *site_xyz, double *site_force);
+/* Host-side copies of the last call's diagnostics. */
+int mgpu_last_info(const mgpu_engine *eng, mgpu_fc_info *fc, mgpu_ew_info *ew);
+
+/* ---- link cells and pair lists (bit-exact with src/force.c:426-569) ----
+ * mgpu_cells: per entry (a site, or a molecule when molpbc) its cell, and the
+ * cell lists in the reference's linked-list order as CSR.
+ * mgpu_pairlist: for every cell the site neighbour list (nab, pbctrans) in the
+ * reference's order, CSR over cells; call with nab == NULL to get the total. */
+int mgpu_cells(mgpu_engine *eng, const double h[9], const double *site, int ld,
+               const double *c_of_m_frac, int *dims3, int *entry_cell, int *cell_start,
+               int *cell_entries);
+int mgpu_pairlist(mgpu_engine *eng, const double h[9], const double *site, int ld,
+                  const double *c_of_m_frac, int strict, double cutoff, long *total,
+                  long *list_start, int *nab, int *pbctrans, int *nfnab);
+
+/* ---- timing helpers for measurement ---- */
+int mgpu_sync(mgpu_engine *eng);
+/* Average device time (ms) of the named kernel over the calls since
+ * mgpu_profile_reset, measured with HIP events on the engine's stream. */
+int mgpu_profile_enable(mgpu_engine *eng, int on);
+int mgpu_profile_reset(mgpu_engine *eng);
+int mgpu_profile_get(const mgpu_engine *eng, const char *name, double *avg_ms, long *calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,67 @@
+"""Builds libmoldy_amd.so (HIP, gfx950) and libmoldyforce.so (C shim) in-tree.
+
+    python -m moldy_amd.build [--jobs N]
+
+Every HIP translation unit is compiled for gfx950 with packed-FP32 code
+generation disabled; objects go to moldy_amd/_build, libraries to moldy_amd/.
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+BUILD = os.path.join(HERE, "_build")
+INC = os.path.join(os.path.dirname(HERE), "include")
+LIB = os.path.join(HERE, "libmoldy_amd.so")
+SHIM = os.path.join(HERE, "libmoldyforce.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
+             "-Wno-unused-variable", "-Wno-unused-function",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + INC, "-I" + CSRC]
+SOURCES = ["mgpu_engine.hip", "mgpu_cells.hip", "mgpu_realspace.hip", "mgpu_ewald.hip",
+           "mgpu_ewald_mixed.hip", "mgpu_glue.hip"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise SystemExit(f"build failed: {cmd[-1]}")
+    return r
+
+
+def build(jobs=8, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in SOURCES:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    shim_src = os.path.join(HOST, "force_shim.c")
+    if os.path.exists(shim_src):
+        tmp = SHIM + ".tmp"
+        _run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-Wall", "-I" + INC, "-o", tmp,
+              shim_src, "-L" + HERE, "-lmoldy_amd", "-Wl,-rpath,$ORIGIN", "-lm"])
+        os.replace(tmp, SHIM)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=8)
+    a = ap.parse_args()
+    build(a.jobs, verbose=True)

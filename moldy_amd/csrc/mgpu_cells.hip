@@ -1,0 +1,345 @@
+// Link-cell binning and site neighbour lists on the GPU.
+//
+// Binning restates fill_cells/cellbin (src/force.c:119-137, 426-486): the
+// fractional coordinate s = h^-1 r is formed in the reference's operation
+// order with FMA contraction off, then floor((s+0.5)*n) with the 8*DBL_EPSILON
+// edge clamp.  Moldy pushes each entry on the head of its cell's linked list,
+// so a cell lists its entries in decreasing entry order; the counting sort
+// below reproduces exactly that order.  The cell-ordered site arrays written
+// here feed the real-space kernel.
+//
+// The site neighbour lists restate site_neighbour_list (src/force.c:521-569):
+// per reference cell, frame type by frame type, the half-shell offsets in
+// list order (own cell excluded for the framework pass), each neighbour
+// cell's entries in list order, and each entry's sites, tagged with the
+// periodic image code 9*(sx+1)+3*(sy+1)+(sz+1).
+#include "mgpu_internal.h"
+
+using namespace mgpu;
+
+#define FLAG_BIN_RANGE 1
+#define FLAG_BIN_ROUND 2
+
+// ----------------------------------------------------------------- scan
+static __global__ void k_scan_block(const int *__restrict__ in, int *__restrict__ out,
+                                    int *__restrict__ bsum, int n) {
+   __shared__ int ws[16];
+   int i = blockIdx.x * 1024 + threadIdx.x;
+   int v = i < n ? in[i] : 0;
+   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   int inc = wave_incl_scan(v, lane);
+   if (lane == 63) ws[w] = inc;
+   __syncthreads();
+   if (w == 0) {
+      int t = lane < 16 ? ws[lane] : 0;
+      int ti = wave_incl_scan(t, lane);
+      if (lane < 16) ws[lane] = ti - t;
+   }
+   __syncthreads();
+   int excl = inc - v + ws[w];
+   if (i < n) out[i] = excl;
+   if (threadIdx.x == 1023) bsum[blockIdx.x] = excl + v;
+}
+
+static __global__ void k_scan_sums(int *__restrict__ bsum, int nb, int *__restrict__ total) {
+   // single block: exclusive scan of up to 1024*k block sums
+   __shared__ int carry;
+   if (threadIdx.x == 0) carry = 0;
+   __syncthreads();
+   for (int base = 0; base < nb; base += 1024) {
+      __shared__ int ws[16];
+      int i = base + threadIdx.x;
+      int v = i < nb ? bsum[i] : 0;
+      int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      int inc = wave_incl_scan(v, lane);
+      if (lane == 63) ws[w] = inc;
+      __syncthreads();
+      if (w == 0) {
+         int t = lane < 16 ? ws[lane] : 0;
+         int ti = wave_incl_scan(t, lane);
+         if (lane < 16) ws[lane] = ti - t;
+      }
+      __syncthreads();
+      int excl = inc - v + ws[w] + carry;
+      if (i < nb) bsum[i] = excl;
+      __syncthreads();
+      if (threadIdx.x == 1023) carry = excl + v;
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) *total = carry;
+}
+
+static __global__ void k_scan_add(int *__restrict__ out, const int *__restrict__ bsum, int n,
+                                  const int *__restrict__ total) {
+   int i = blockIdx.x * 1024 + threadIdx.x;
+   if (i < n) out[i] += bsum[blockIdx.x];
+   if (i == 0) out[n] = *total;
+}
+
+void mgpu::device_scan_int(mgpu_engine *e, const int *in, int *out, int n) {
+   int nb = (n + 1023) / 1024;
+   e->d_scan_tmp.reserve((size_t)nb + 2);
+   int *bsum = e->d_scan_tmp.p, *total = e->d_scan_tmp.p + nb;
+   if (nb == 0) nb = 1;
+   k_scan_block<<<nb, 1024, 0, e->stream>>>(in, out, bsum, n);
+   k_scan_sums<<<1, 1024, 0, e->stream>>>(bsum, (n + 1023) / 1024, total);
+   k_scan_add<<<nb, 1024, 0, e->stream>>>(out, bsum, n, total);
+}
+
+// ---------------------------------------------------------------- binning
+#pragma clang fp contract(off)
+__device__ __forceinline__ int cellbin_dev(double rc, int nc, double fnc, double eps, int *flag) {
+   if (rc < -0.5 + eps || rc >= 0.5 - eps) {
+      if (rc < -0.5 + eps && rc >= -0.5 - eps)
+         rc = -0.5;
+      else if (rc >= 0.5 - eps && rc <= 0.5 + eps)
+         rc = 0.5 - eps;
+      else
+         *flag |= FLAG_BIN_RANGE;
+   }
+   int ib = (int)floor((rc + 0.5) * fnc);
+   if (ib >= nc || ib < 0) {
+      *flag |= FLAG_BIN_ROUND;
+      ib = ib < 0 ? 0 : nc - 1;
+   }
+   return ib;
+}
+
+struct BinArgs {
+   double hi[9];
+   int nx, ny, nz, ne, nsp;
+   double eps;
+};
+
+static __global__ void k_bin(BinArgs a, const double *__restrict__ site, const double *__restrict__ cm,
+                             const int *__restrict__ e_isite, const int *__restrict__ e_num,
+                             const int *__restrict__ e_mol, int *__restrict__ e_cell,
+                             int *__restrict__ e_slot, int *__restrict__ c_count,
+                             int *__restrict__ c_nsites, int *__restrict__ flags) {
+   int e = blockIdx.x * blockDim.x + threadIdx.x;
+   if (e >= a.ne) return;
+   double s0, s1, s2;
+   int m = e_mol[e];
+   if (m < 0) {
+      int is = e_isite[e];
+      double r0 = site[is], r1 = site[a.nsp + is], r2 = site[2 * a.nsp + is];
+      // mat_vec_mul, src/matrix.c:81-83 (left-to-right, no contraction)
+      s0 = a.hi[0] * r0 + a.hi[1] * r1 + a.hi[2] * r2;
+      s1 = a.hi[3] * r0 + a.hi[4] * r1 + a.hi[5] * r2;
+      s2 = a.hi[6] * r0 + a.hi[7] * r1 + a.hi[8] * r2;
+   } else {
+      s0 = cm[3 * m];
+      s1 = cm[3 * m + 1];
+      s2 = cm[3 * m + 2];
+   }
+   int flag = 0;
+   int c = cellbin_dev(s2, a.nz, (double)a.nz, a.eps, &flag) +
+           a.nz * (cellbin_dev(s1, a.ny, (double)a.ny, a.eps, &flag) +
+                   a.ny * cellbin_dev(s0, a.nx, (double)a.nx, a.eps, &flag));
+   if (flag) atomicOr(flags, flag);
+   e_cell[e] = c;
+   e_slot[e] = atomicAdd(&c_count[c], 1);
+   atomicAdd(&c_nsites[c], e_num[e]);
+}
+#pragma clang fp contract(on)
+
+static __global__ void k_place(int ne, const int *__restrict__ e_cell, const int *__restrict__ e_slot,
+                               const int *__restrict__ c_start, int *__restrict__ c_entries) {
+   int e = blockIdx.x * blockDim.x + threadIdx.x;
+   if (e >= ne) return;
+   c_entries[c_start[e_cell[e]] + e_slot[e]] = e;
+}
+
+struct FinArgs {
+   double h[9];
+   int nx, ny, nz, ncells, nsp;
+};
+
+// One thread per cell: order the cell's entries as Moldy's list (descending
+// entry index), then lay the entries' sites out in that order.
+static __global__ void k_cell_finalize(FinArgs a, const int *__restrict__ c_start,
+                                       int *__restrict__ c_entries, const int *__restrict__ c_site_start,
+                                       const int *__restrict__ e_isite, const int *__restrict__ e_num,
+                                       const double *__restrict__ site, const double *__restrict__ chg,
+                                       const int *__restrict__ stype, const int *__restrict__ smol,
+                                       const int *__restrict__ sfw, double4 *__restrict__ cpos,
+                                       float4 *__restrict__ cposf, int2 *__restrict__ cmeta,
+                                       int *__restrict__ corig) {
+   int c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= a.ncells) return;
+   int b = c_start[c], n = c_start[c + 1] - b;
+   // insertion sort, descending (cells hold few entries)
+   for (int i = 1; i < n; i++) {
+      int v = c_entries[b + i], j = i - 1;
+      while (j >= 0 && c_entries[b + j] < v) {
+         c_entries[b + j + 1] = c_entries[b + j];
+         j--;
+      }
+      c_entries[b + j + 1] = v;
+   }
+   int ix = c / (a.ny * a.nz), iy = c / a.nz - a.ny * ix, iz = c - a.nz * (iy + a.ny * ix);
+   double f0 = (double)ix / a.nx - 0.5, f1 = (double)iy / a.ny - 0.5, f2 = (double)iz / a.nz - 0.5;
+   double o0 = a.h[0] * f0 + a.h[1] * f1 + a.h[2] * f2;
+   double o1 = a.h[3] * f0 + a.h[4] * f1 + a.h[5] * f2;
+   double o2 = a.h[6] * f0 + a.h[7] * f1 + a.h[8] * f2;
+   int pos = c_site_start[c];
+   for (int i = 0; i < n; i++) {
+      int en = c_entries[b + i];
+      int is0 = e_isite[en], nn = e_num[en];
+      for (int k = 0; k < nn; k++, pos++) {
+         int is = is0 + k;
+         double x = site[is], y = site[a.nsp + is], z = site[2 * a.nsp + is], q = chg[is];
+         cpos[pos] = make_double4(x, y, z, q);
+         cposf[pos] = make_float4((float)(x - o0), (float)(y - o1), (float)(z - o2), (float)q);
+         cmeta[pos] = make_int2(stype[is] | (sfw[is] << 8), smol[is]);
+         corig[pos] = is;
+      }
+   }
+}
+
+// Bin the current site coordinates (e->d_site) and build the cell-ordered
+// arrays.  Requires e->nx/ny/nz set and e->hinv valid.
+void mgpu::run_binning(mgpu_engine *e, bool) {
+   ProfScope ps(e, "binning");
+   int nc = e->ncells;
+   e->d_entry_cell.reserve(e->nentries);
+   e->d_entry_slot.reserve(e->nentries);
+   e->d_cell_count.reserve(nc + 1);
+   e->d_cell_nsites.reserve(nc + 1);
+   e->d_cell_start.reserve(nc + 1);
+   e->d_cell_site_start.reserve(nc + 1);
+   e->d_cell_entries.reserve(e->nentries + 1);
+   e->d_cpos.reserve(e->nsites + 1);
+   e->d_cposf.reserve(e->nsites + 1);
+   e->d_cmeta.reserve(e->nsites + 1);
+   e->d_corig.reserve(e->nsites + 1);
+   HIPCHK(hipMemsetAsync(e->d_cell_count.p, 0, sizeof(int) * (nc + 1), e->stream));
+   HIPCHK(hipMemsetAsync(e->d_cell_nsites.p, 0, sizeof(int) * (nc + 1), e->stream));
+   BinArgs a;
+   for (int i = 0; i < 9; i++) a.hi[i] = e->hinv[i];
+   a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.ne = e->nentries; a.nsp = e->nsp;
+   a.eps = 8.0 * 2.220446049250313e-16;   // 8 * precision(), src/force.c:437
+   int nb = (e->nentries + 255) / 256;
+   k_bin<<<nb, 256, 0, e->stream>>>(a, e->d_site.p, e->d_cm.p, e->d_entry_isite.p,
+                                    e->d_entry_num.p, e->d_entry_mol.p, e->d_entry_cell.p,
+                                    e->d_entry_slot.p, e->d_cell_count.p, e->d_cell_nsites.p,
+                                    e->d_iflags.p);
+   device_scan_int(e, e->d_cell_count.p, e->d_cell_start.p, nc);
+   device_scan_int(e, e->d_cell_nsites.p, e->d_cell_site_start.p, nc);
+   k_place<<<nb, 256, 0, e->stream>>>(e->nentries, e->d_entry_cell.p, e->d_entry_slot.p,
+                                      e->d_cell_start.p, e->d_cell_entries.p);
+   FinArgs f;
+   for (int i = 0; i < 9; i++) f.h[i] = e->h[i];
+   f.nx = e->nx; f.ny = e->ny; f.nz = e->nz; f.ncells = nc; f.nsp = e->nsp;
+   k_cell_finalize<<<(nc + 255) / 256, 256, 0, e->stream>>>(
+       f, e->d_cell_start.p, e->d_cell_entries.p, e->d_cell_site_start.p, e->d_entry_isite.p,
+       e->d_entry_num.p, e->d_site.p, e->d_chg_site.p, e->d_site_type.p, e->d_site_mol.p,
+       e->d_site_fw.p, e->d_cpos.p, e->d_cposf.p, e->d_cmeta.p, e->d_corig.p);
+}
+
+// ------------------------------------------------------ site neighbour lists
+struct PLArgs {
+   int nx, ny, nz, ncells, noff, nftypes;
+};
+
+// One wavefront per reference cell.  Lanes take offsets; each lane counts
+// the sites its neighbour cell contributes for the current frame type, a
+// wave prefix sum places them, and (fill mode) the lane writes them.
+template <bool FILL>
+static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *__restrict__ off,
+                                                         const int *__restrict__ c_start,
+                                                         const int *__restrict__ c_entries,
+                                                         const int *__restrict__ e_isite,
+                                                         const int *__restrict__ e_num,
+                                                         const int *__restrict__ e_ftype,
+                                                         int *__restrict__ count,
+                                                         const int *__restrict__ lstart,
+                                                         int *__restrict__ nab, int *__restrict__ pbct,
+                                                         int *__restrict__ nfnab) {
+   int lane = threadIdx.x & 63;
+   int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+   if (c >= a.ncells) return;
+   int ix = c / (a.ny * a.nz), iy = c / a.nz - a.ny * ix, iz = c - a.nz * (iy + a.ny * ix);
+   int base = FILL ? lstart[c] : 0;
+   int nn = 0;
+   for (int ft = 0; ft < a.nftypes; ft++) {
+      for (int o0 = (ft == 0 ? 0 : 1); o0 < a.noff; o0 += 64) {
+         int o = o0 + lane, cnt = 0, jc = 0, code = 0;
+         if (o < a.noff) {
+            int4 d = off[o];
+            int ex = ix + d.x, ey = iy + d.y, ez = iz + d.z;
+            int sx = ex < 0 ? -1 : (ex >= a.nx ? 1 : 0);
+            int sy = ey < 0 ? -1 : (ey >= a.ny ? 1 : 0);
+            int sz = ez < 0 ? -1 : (ez >= a.nz ? 1 : 0);
+            jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
+            code = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
+            for (int q = c_start[jc]; q < c_start[jc + 1]; q++) {
+               int en = c_entries[q];
+               if (e_ftype[en] == ft) cnt += e_num[en];
+            }
+         }
+         int inc = wave_incl_scan(cnt, lane);
+         if (FILL && cnt) {
+            int p = base + nn + inc - cnt;
+            for (int q = c_start[jc]; q < c_start[jc + 1]; q++) {
+               int en = c_entries[q];
+               if (e_ftype[en] != ft) continue;
+               int s0 = e_isite[en], m = e_num[en];
+               for (int k = 0; k < m; k++, p++) {
+                  nab[p] = s0 + k;
+                  pbct[p] = code;
+               }
+            }
+         }
+         nn += __shfl(inc, 63, 64);
+      }
+      if (FILL && lane == 0) nfnab[2 * c + ft] = nn;
+   }
+   if (FILL && lane == 0 && a.nftypes == 1) nfnab[2 * c + 1] = nn;
+   if (!FILL && lane == 0) count[c] = nn;
+}
+
+void mgpu::run_pairlist_count(mgpu_engine *e, int, double, const int *offs, int noff,
+                              long *total, long *list_start_host) {
+   ProfScope ps(e, "pairlist_count");
+   std::vector<int4> h(noff);
+   for (int i = 0; i < noff; i++) h[i] = make_int4(offs[3 * i], offs[3 * i + 1], offs[3 * i + 2], 0);
+   e->d_off_pl.reserve(noff + 1);
+   HIPCHK(hipMemcpyAsync(e->d_off_pl.p, h.data(), sizeof(int4) * noff, hipMemcpyHostToDevice, e->stream));
+   int nc = e->ncells;
+   e->d_pl_count.reserve(nc + 1);
+   e->d_pl_start.reserve(nc + 1);
+   PLArgs a{e->nx, e->ny, e->nz, nc, noff, e->n_frame_types};
+   k_pairlist<false><<<(nc + 3) / 4, 256, 0, e->stream>>>(a, e->d_off_pl.p, e->d_cell_start.p,
+                                                          e->d_cell_entries.p, e->d_entry_isite.p,
+                                                          e->d_entry_num.p, e->d_entry_ftype.p,
+                                                          e->d_pl_count.p, nullptr, nullptr, nullptr,
+                                                          nullptr);
+   device_scan_int(e, e->d_pl_count.p, e->d_pl_start.p, nc);
+   std::vector<int> hs(nc + 1);
+   HIPCHK(hipMemcpyAsync(hs.data(), e->d_pl_start.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   for (int i = 0; i <= nc; i++) list_start_host[i] = hs[i];
+   *total = hs[nc];
+}
+
+void mgpu::run_pairlist_fill(mgpu_engine *e, const int *, int noff, long total, int *nab,
+                             int *pbctrans, int *nfnab) {
+   ProfScope ps(e, "pairlist_fill");
+   int nc = e->ncells;
+   mgpu::DBuf<int> dn, dp, df;
+   dn.reserve(total + 1);
+   dp.reserve(total + 1);
+   df.reserve(2 * (size_t)nc + 2);
+   PLArgs a{e->nx, e->ny, e->nz, nc, noff, e->n_frame_types};
+   k_pairlist<true><<<(nc + 3) / 4, 256, 0, e->stream>>>(
+       a, e->d_off_pl.p, e->d_cell_start.p, e->d_cell_entries.p, e->d_entry_isite.p,
+       e->d_entry_num.p, e->d_entry_ftype.p, nullptr, e->d_pl_start.p, dn.p, dp.p, df.p);
+   if (nab) HIPCHK(hipMemcpyAsync(nab, dn.p, sizeof(int) * total, hipMemcpyDeviceToHost, e->stream));
+   if (pbctrans) HIPCHK(hipMemcpyAsync(pbctrans, dp.p, sizeof(int) * total, hipMemcpyDeviceToHost, e->stream));
+   if (nfnab) HIPCHK(hipMemcpyAsync(nfnab, df.p, sizeof(int) * 2 * nc, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   dn.free();
+   dp.free();
+   df.free();
+}

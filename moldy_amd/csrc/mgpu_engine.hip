@@ -1,0 +1,829 @@
+// Engine lifetime, host-side geometry and the C ABI (include/moldy_amd.h).
+//
+// Host work per call is O(offsets + k rows): the cell partition
+// (src/force.c:1153-1157), the half-shell neighbour offsets
+// (neighbour_list / strict_neighbour_list, src/force.c:167-421) and the
+// reciprocal-lattice set (src/ewald.c:309-311, 435-461), each cached on the
+// inputs that determine it.  The one-off constants the reference keeps in
+// statics -- the intramolecular energy (force.c:1158-1168), the Ewald self
+// and sheet energies (ewald.c:367-425) and the distant-potential constants
+// (accel.c:293-329) -- are computed once per system.
+#include "mgpu_internal.h"
+
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+
+using namespace mgpu;
+
+namespace {
+constexpr double PI_D = 3.14159265358979323846;
+
+double err_fn_h(double x) {   // src/auxil.c:586-593
+   const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741,
+                A4 = -1.453152027, A5 = 1.061405429;
+   double t = 1.0 / (1.0 + P * x);
+   if (x >= 0.0) return 1.0 - t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5)))) * exp(-x * x);
+   return -err_fn_h(-x);
+}
+
+// Energy of one pair (the u of kernel(), src/kernel.c:188-460): used for the
+// one-off intramolecular correction and the distant-potential constant.
+double pair_energy(int pt, double alpha, double rsq, double qq, const double *p) {
+   const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741,
+                A4 = -1.453152027, A5 = 1.061405429;
+   double r = sqrt(rsq), t = 0;
+   if (alpha > 0.0) {
+      double ar = alpha * r, tt = 1.0 / (1.0 + P * ar);
+      double et = qq * exp(-ar * ar);
+      t = tt * (A1 + tt * (A2 + tt * (A3 + tt * (A4 + tt * A5)))) * et / r;
+   }
+   double r2r = 1.0 / rsq, r4, r6, r8, r12, e1, e2, e3;
+   switch (pt) {
+   case 0:
+      r6 = p[1] * p[1] * r2r; r6 = r6 * r6 * r6; r12 = r6 * r6;
+      return t + p[0] * (r12 - r6);
+   case 1:
+      return t - p[0] * r2r * r2r * r2r + p[1] * exp(-p[2] * r);
+   case 2:
+      return t + p[0] * exp(-p[1] * r) - p[2] * exp(-p[3] * r);
+   case 3:
+      r4 = r2r * r2r; r6 = r2r * r4; r8 = p[5] * r4 * r4; r12 = p[2] * r6 * r6;
+      return t + p[0] * exp(-p[1] * r) + r12 - p[3] * r4 - p[4] * r6 - r8;
+   case 4:
+      r4 = r2r * r2r; r6 = r2r * r4;
+      return t + p[0] * r4 + p[1] * r6 + p[2] * r6 * r6;
+   case 6:
+      e1 = p[0] * exp((p[1] - r) * p[2]);
+      e2 = p[4] * exp(-2.0 * p[5] * (r - p[6]));
+      e3 = -p[4] * 2.0 * exp(-p[5] * (r - p[6]));
+      return t + e1 - p[3] * r2r * r2r * r2r + e2 + e3;
+   default:
+      throw StatusError{MGPU_E_PTYPE, "unknown potential type"};
+   }
+}
+
+// dist_pot, src/kernel.c:103-151
+double dist_pot_h(const double *p, double rc, int pt) {
+   auto sq = [](double x) { return x * x; };
+   auto cu = [](double x) { return x * x * x; };
+   switch (pt) {
+   case 0: return p[0] * cu(sq(p[1]) / rc) / 3.0;
+   case 1:
+      if (p[2] > 1e-7)
+         return p[0] / (3.0 * cu(rc)) - p[1] * exp(-p[2] * rc) * (sq(rc) / p[2] + 2 * rc / sq(p[2]) + 2.0 / cu(p[2]));
+      return p[0] / (3.0 * cu(rc));
+   case 2:
+      if (p[3] > 1e-7)
+         return p[2] * (sq(rc) / p[3] + 2 * rc / sq(p[3]) + 2.0 / cu(p[3])) * exp(-p[3] * rc);
+      return 0.0;
+   case 3: {
+      double tail = -p[2] / (9.0 * cu(cu(rc))) + p[3] / rc + p[4] / (3.0 * cu(rc)) + p[5] / (5.0 * sq(rc) * cu(rc));
+      if (p[1] > 1e-7) return -p[0] * exp(-p[1] * rc) * (sq(rc) / p[1] + 2 * rc / sq(p[1]) + 2.0 / cu(p[1])) + tail;
+      return tail;
+   }
+   case 6:
+      if (p[5] != 0.0)
+         return p[3] / (3.0 * cu(rc)) + 2.0 * p[4] * (sq(rc) / p[5] + 2 * rc / sq(p[5]) + 2.0 / cu(p[5])) *
+                                            exp(-p[5] * (rc - p[6]));
+      return p[3] / (3.0 * cu(rc));
+   case 4: return -p[0] / rc - p[1] / cu(rc) / 3.0 - p[2] / cu(cu(rc)) / 9.0;
+   default: return 0.0;
+   }
+}
+
+void transpose3(const double a[9], double b[9]) {
+   double t[9];
+   memcpy(t, a, sizeof t);
+   for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) b[3 * i + j] = t[3 * j + i];
+}
+void matmul3(const double a[9], const double b[9], double c[9]) {
+   double r[9];
+   for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+         r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+   memcpy(c, r, sizeof r);
+}
+double col_norm(const double m[9], int c) { return sqrt(m[c] * m[c] + m[3 + c] * m[3 + c] + m[6 + c] * m[6 + c]); }
+
+}  // namespace
+
+#pragma clang fp contract(off)
+double mgpu::host_det(const double a[9]) {   // src/matrix.c:159-168
+   double d = 0.0;
+   for (int i = 0, j = 1, k = 2; i < 3; i++, j = (j + 1) % 3, k = (k + 1) % 3)
+      d += a[i] * (a[3 + j] * a[6 + k] - a[3 + k] * a[6 + j]);
+   return d;
+}
+void mgpu::host_invert(const double a[9], double b[9]) {   // src/matrix.c:171-185
+   double r = 1.0 / host_det(a);
+   for (int i = 0, j = 1, k = 2; i < 3; i++, j = (j + 1) % 3, k = (k + 1) % 3)
+      for (int l = 0, m = 1, n = 2; l < 3; l++, m = (m + 1) % 3, n = (n + 1) % 3)
+         b[3 * l + i] = r * (a[3 * j + m] * a[3 * k + n] - a[3 * j + n] * a[3 * k + m]);
+}
+
+// Half-shell offsets in the reference's order; returns count or -status.
+static int neighbour_offsets(const double h[9], double cutoff, int nx, int ny, int nz, int strict,
+                             std::vector<int> &out) {
+   double G[9], htr[9], htrinv[9], sv[3];
+   transpose3(h, htr);
+   matmul3(htr, h, G);
+   host_invert(htr, htrinv);
+   int mx = (int)ceil(cutoff * nx * col_norm(htrinv, 0));
+   int my = (int)ceil(cutoff * ny * col_norm(htrinv, 1));
+   int mz = (int)ceil(cutoff * nz * col_norm(htrinv, 2));
+   out.clear();
+   auto dist_of = [&](int ix, int iy, int iz) {
+      sv[0] = (double)ix / nx; sv[1] = (double)iy / ny; sv[2] = (double)iz / nz;
+      double d = 0.0;
+      for (int i = 0; i < 3; i++)
+         for (int j = 0; j < 3; j++) d += sv[i] * G[3 * i + j] * sv[j];
+      return d;
+   };
+   auto bad = [&](int ix, int iy, int iz) {
+      return ix > nx || ix < -nx || iy > ny || iy < -ny || iz > nz || iz < -nz;
+   };
+   if (!strict) {
+      for (int ix = 0; ix < mx; ix++)
+         for (int iy = (ix == 0 ? 0 : -my); iy < my; iy++)
+            for (int iz = (ix == 0 && iy == 0 ? 0 : -mz); iz < mz; iz++)
+               if (dist_of(ix, iy, iz) < cutoff * cutoff) {
+                  if (bad(ix, iy, iz)) return -MGPU_E_CUTOFF;
+                  out.push_back(ix); out.push_back(iy); out.push_back(iz);
+               }
+      return (int)out.size() / 3;
+   }
+   int sy = 2 * my + 2, sz = 2 * mz + 2;
+   std::vector<unsigned char> map((size_t)(mx + 1) * sy * sz, 0);
+   auto M = [&](int a, int b, int c) -> unsigned char & {
+      return map[((size_t)a * sy + (b + my + 1)) * sz + (c + mz + 1)];
+   };
+   for (int ix = 0; ix < mx; ix++)
+      for (int iy = (ix == 0 ? 0 : -my); iy < my; iy++)
+         for (int iz = (ix == 0 && iy == 0 ? 0 : -mz); iz < mz; iz++)
+            if (dist_of(ix, iy, iz) < cutoff * cutoff)
+               for (int i = 0; i <= 1; i++)
+                  for (int j = -1; j <= 1; j++)
+                     for (int k = -1; k <= 1; k++) M(ix + i, iy + j, iz + k) = 1;
+   int nxyz[3] = {nx, ny, nz}, mxyz[3] = {mx, my, mz};
+   for (int a = 0; a < 3; a++) {   // face-face contacts, src/force.c:338-383
+      int b = (a + 1) % 3, c = (b + 1) % 3, fc[4][3];
+      double proj[3] = {0, 0, 0}, modabc = 0.0;
+      for (int i = 0; i < 3; i++) {
+         modabc += htrinv[3 * i + a];
+         proj[i] += htrinv[3 * i + a] * htrinv[3 * i + (a + i) % 3];
+      }
+      for (int i = 0; i < 3; i++) proj[i] *= (mxyz[a] - 1) * nxyz[i] / (nxyz[a] * modabc);
+      for (int i = 0; i < 3; i++) fc[0][i] = fc[1][i] = fc[2][i] = fc[3][i] = (int)floor(proj[i]);
+      fc[0][a] = fc[1][a] = fc[2][a] = fc[3][a] = mxyz[a];
+      fc[1][b] = fc[3][b] = (int)ceil(proj[b]);
+      fc[2][c] = fc[3][c] = (int)ceil(proj[c]);
+      for (int i = 0; i < 4; i++)
+         if (fc[i][0] < 0)
+            for (int j = 0; j < 3; j++) fc[i][j] = -fc[i][j];
+      for (int i = 0; i < 4; i++) {
+         if (fc[i][0] < 0 || fc[i][0] > mx || fc[i][1] < -my - 1 || fc[i][1] > my || fc[i][2] < -mz - 1 ||
+             fc[i][2] > mz)
+            return -MGPU_E_CUTOFF;
+         M(fc[i][0], fc[i][1], fc[i][2]) = 1;
+      }
+   }
+   for (int ix = 0; ix <= mx; ix++)
+      for (int iy = (ix == 0 ? 0 : -my - 1); iy <= my; iy++)
+         for (int iz = (ix == 0 && iy == 0 ? 0 : -mz - 1); iz <= mz; iz++)
+            if (M(ix, iy, iz)) {
+               if (bad(ix, iy, iz)) return -MGPU_E_CUTOFF;
+               out.push_back(ix); out.push_back(iy); out.push_back(iz);
+            }
+   return (int)out.size() / 3;
+}
+#pragma clang fp contract(on)
+
+namespace {
+
+void set_geometry(mgpu_engine *e, const double h[9]) {
+#pragma clang fp contract(off)
+   memcpy(e->h, h, 9 * sizeof(double));
+   host_invert(e->h, e->hinv);
+   double sub = e->subcell > 0.0 ? e->subcell : e->cutoff / 5.0;
+   int nx = (int)(h[0] / sub + 0.5), ny = (int)(h[4] / sub + 0.5), nz = (int)(h[8] / sub + 0.5);
+   if (nx < 1 || ny < 1 || nz < 1) throw StatusError{MGPU_E_ARG, "cell too small for the subcell size"};
+   double key[13] = {h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8], e->cutoff, (double)e->strict,
+                     (double)(nx * 1000000 + ny * 1000 + nz), 1.0};
+   if (memcmp(key, e->off_key, sizeof key) != 0 || e->n_full == 0) {
+      std::vector<int> off;
+      int n = neighbour_offsets(h, e->cutoff, nx, ny, nz, e->strict, off);
+      if (n < 0) throw StatusError{-n, "cutoff radius exceeds the image-cell range (CUTOFF)"};
+      e->nx = nx; e->ny = ny; e->nz = nz; e->ncells = nx * ny * nz;
+      e->off_half = off;
+      e->n_half = n;
+      // full shell: the half shell then the mirror of every non-zero offset
+      std::vector<int4> full;
+      std::vector<float4> D;
+      for (int s = 0; s < 2; s++)
+         for (int i = (s ? 1 : 0); i < n; i++) {
+            int dx = off[3 * i], dy = off[3 * i + 1], dz = off[3 * i + 2];
+            if (s) { dx = -dx; dy = -dy; dz = -dz; }
+            full.push_back(make_int4(dx, dy, dz, 0));
+            double f0 = (double)dx / nx, f1 = (double)dy / ny, f2 = (double)dz / nz;
+            D.push_back(make_float4((float)(h[0] * f0 + h[1] * f1 + h[2] * f2),
+                                    (float)(h[3] * f0 + h[4] * f1 + h[5] * f2),
+                                    (float)(h[6] * f0 + h[7] * f1 + h[8] * f2), 0.f));
+         }
+      e->n_full = (int)full.size();
+      e->d_off.reserve(full.size() + 1);
+      e->d_offD.reserve(full.size() + 1);
+      HIPCHK(hipMemcpyAsync(e->d_off.p, full.data(), sizeof(int4) * full.size(), hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(e->d_offD.p, D.data(), sizeof(float4) * D.size(), hipMemcpyHostToDevice, e->stream));
+      memcpy(e->off_key, key, sizeof key);
+   }
+   if (e->alpha > 1e-7) {
+      double kkey[10] = {h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8], e->k_cutoff};
+      if (memcmp(kkey, e->kset_key, sizeof kkey) != 0 || e->nk == 0) {
+         double hp[9];
+         for (int i = 0; i < 9; i++) hp[i] = 2 * PI_D * e->hinv[i];
+         double kc = e->k_cutoff, kcsq = kc * kc;
+         e->hmax = (int)floor(kc / (2 * PI_D) * h[0]);
+         e->kmax = (int)floor(kc / (2 * PI_D) * sqrt(h[1] * h[1] + h[4] * h[4]));
+         e->lmax = (int)floor(kc / (2 * PI_D) * sqrt(h[2] * h[2] + h[5] * h[5] + h[8] * h[8]));
+         // the S(k) kernel stages exp(i l c*.r), l = 0..lmax, for 32 sites in 64 KB of LDS
+         if (e->lmax > 80) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large (lmax > 80)"};
+         e->rows.clear();
+         e->hkl.clear();
+         std::vector<int> krow;
+         int nk = 0;
+         for (int hh = 0; hh <= e->hmax; hh++)
+            for (int kk = (hh == 0 ? 0 : -e->kmax); kk <= e->kmax; kk++) {
+               double kx = hh * hp[0] + kk * hp[3];
+               double ky = hh * hp[1] + kk * hp[4];
+               double kzt = hh * hp[2] + kk * hp[5];
+               double ksq = kx * kx + ky * ky;
+               KRow R{hh, kk, 0, 0, nk, 0, kx, ky, kzt};
+               for (int ll = (hh == 0 && kk == 0 ? 1 : -e->lmax); ll <= e->lmax; ll++) {
+                  double kz = kzt + ll * hp[8];
+                  if (kz * kz + ksq < kcsq) {
+                     if (R.nl == 0) R.l0 = ll;
+                     else if (ll != R.l0 + R.nl) {
+                        // not contiguous: close this row, start another
+                        e->rows.push_back(R);
+                        R = KRow{hh, kk, ll, 0, nk, 0, kx, ky, kzt};
+                     }
+                     R.nl++;
+                     e->hkl.push_back(hh); e->hkl.push_back(kk); e->hkl.push_back(ll);
+                     krow.push_back((int)e->rows.size());
+                     nk++;
+                  }
+               }
+               if (R.nl) e->rows.push_back(R);
+            }
+         e->nk = nk;
+         e->nrows = (int)e->rows.size();
+         e->d_rows.reserve(e->rows.size() + 1);
+         e->d_krow.reserve(krow.size() + 1);
+         HIPCHK(hipMemcpyAsync(e->d_rows.p, e->rows.data(), sizeof(KRow) * e->rows.size(), hipMemcpyHostToDevice,
+                               e->stream));
+         HIPCHK(hipMemcpyAsync(e->d_krow.p, krow.data(), sizeof(int) * krow.size(), hipMemcpyHostToDevice,
+                               e->stream));
+         HIPCHK(hipStreamSynchronize(e->stream));
+         memcpy(e->kset_key, kkey, sizeof kkey);
+      }
+   }
+}
+
+void compute_statics(mgpu_engine *e, const double *chg) {
+   // eintra, src/force.c:654-668 + 1158-1168 (charges of each species' first molecule)
+   double eintra = 0.0;
+   int isite = 0;
+   for (int s = 0; s < e->nspecies; s++) {
+      int ns = e->spec_nsites[s];
+      const int *id = e->spec_site_id.data() + e->spec_off[s];
+      const double *pf = e->spec_pf.data() + 3 * e->spec_off[s];
+      if (!e->spec_fw[s]) {
+         double ei = 0.0;
+         for (int b = 0; b < ns; b++)
+            for (int a = b + 1; a < ns; a++) {
+               double d2 = 0;
+               for (int c = 0; c < 3; c++) d2 += (pf[3 * a + c] - pf[3 * b + c]) * (pf[3 * a + c] - pf[3 * b + c]);
+               double d = sqrt(d2);
+               ei += pair_energy(e->ptype, e->alpha, d * d, chg[isite + a] * chg[isite + b],
+                                 e->potpar.data() + MGPU_NPOTP * (id[b] * e->max_id + id[a]));
+            }
+         eintra += e->spec_nmols[s] * ei;
+      }
+      isite += e->spec_nmols[s] * ns;
+   }
+   e->eintra = eintra;
+   // self and sheet energy, src/ewald.c:367-425
+   double self = 0.0, sheet = 0.0, sq = 0.0, sqsq = 0.0;
+   int ssite = 0, s = 0;
+   for (; s < e->nspecies && !e->spec_fw[s]; s++) {
+      int ns = e->spec_nsites[s];
+      const double *pf = e->spec_pf.data() + 3 * e->spec_off[s];
+      double intra = 0.0;
+      for (int a = 0; a < ns; a++)
+         for (int b = a + 1; b < ns; b++) {
+            double d2 = 0;
+            for (int c = 0; c < 3; c++) d2 += (pf[3 * a + c] - pf[3 * b + c]) * (pf[3 * a + c] - pf[3 * b + c]);
+            double r = sqrt(d2);
+            intra += chg[ssite + a] * chg[ssite + b] * err_fn_h(e->alpha * r) / r;
+         }
+      self += e->spec_nmols[s] * intra;
+      ssite += ns * e->spec_nmols[s];
+   }
+   int i = 0;
+   for (; i < ssite; i++) { sq += chg[i]; sqsq += chg[i] * chg[i]; }
+   if (e->alpha > 0) self += e->alpha / sqrt(PI_D) * sqsq;
+   double sqxf = sq;
+   for (; i < e->nsites; i++) sq += chg[i];
+   if (e->alpha > 0) {
+      if (s != e->nspecies) sheet = PI_D * (sq - sqxf) * (sq - sqxf) / (2.0 * e->alpha * e->alpha);
+      const double conv_q = 4.07497263794495e-14 * 1.e-3 * 1.05482230112e-05 / 1.60217733e-19;
+      if (fabs(sq) * conv_q > 1.0e-5) sheet -= PI_D * sq * sq / (2.0 * e->alpha * e->alpha);
+   }
+   e->self_energy = self;
+   e->sheet_energy = sheet;
+   // distant constants, src/accel.c:293-329
+   std::vector<long> count(e->max_id, 0);
+   for (int sp = 0; sp < e->nspecies; sp++)
+      for (int is = 0; is < e->spec_nsites[sp]; is++) count[e->spec_site_id[e->spec_off[sp] + is]] += e->spec_nmols[sp];
+   double c0 = 0.0, c1 = 0.0;
+   for (int a = 1; a < e->max_id; a++)
+      for (int b = 1; b < e->max_id; b++) {
+         const double *p = e->potpar.data() + MGPU_NPOTP * (a + e->max_id * b);
+         double dp = 2 * PI_D * count[a] * count[b] * dist_pot_h(p, e->cutoff, e->ptype);
+         c0 -= dp;
+         c1 -= dp;
+         c1 += 2.0 / 3.0 * PI_D * count[a] * count[b] * e->cutoff * e->cutoff * e->cutoff *
+               pair_energy(e->ptype, e->alpha, e->cutoff * e->cutoff, 0.0, p);
+      }
+   e->dist = c0;
+   e->distp = c1;
+   e->statics_done = true;
+}
+
+int fail(mgpu_engine *e, int code, const std::string &msg) {
+   if (e) e->err = msg;
+   return code;
+}
+
+template <class F> int guarded(mgpu_engine *e, F &&f) {
+   if (!e) return MGPU_E_ARG;
+   try {
+      return f();
+   } catch (const HipError &h) {
+      return fail(e, MGPU_E_HIP, std::string("HIP error ") + hipGetErrorString(h.err) + " in " + h.what +
+                                     " (line " + std::to_string(h.line) + ")");
+   } catch (const StatusError &s) {
+      return fail(e, s.code, s.msg);
+   } catch (const std::exception &x) {
+      return fail(e, MGPU_E_ARG, x.what());
+   }
+}
+
+void upload_sites(mgpu_engine *e, const double *site, int ld) {
+   e->d_site.reserve(3 * (size_t)e->nsp);
+   for (int c = 0; c < 3; c++)
+      HIPCHK(hipMemcpyAsync(e->d_site.p + (size_t)c * e->nsp, site + (size_t)c * ld, sizeof(double) * e->nsites,
+                            hipMemcpyHostToDevice, e->stream));
+}
+
+void upload_forces(mgpu_engine *e, const double *f, int ld) {
+   e->d_force_site.reserve(3 * (size_t)e->nsp);
+   for (int c = 0; c < 3; c++)
+      HIPCHK(hipMemcpyAsync(e->d_force_site.p + (size_t)c * e->nsp, f + (size_t)c * ld, sizeof(double) * e->nsites,
+                            hipMemcpyHostToDevice, e->stream));
+}
+
+void download_forces(mgpu_engine *e, double *f, int ld) {
+   for (int c = 0; c < 3; c++)
+      HIPCHK(hipMemcpyAsync(f + (size_t)c * ld, e->d_force_site.p + (size_t)c * e->nsp, sizeof(double) * e->nsites,
+                            hipMemcpyDeviceToHost, e->stream));
+}
+
+void check_flags(mgpu_engine *e, int *tooclose_out) {
+   int fl[34];
+   HIPCHK(hipMemcpyAsync(fl, e->d_iflags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   if (tooclose_out) memcpy(tooclose_out, fl + 1, sizeof(int) * 33);
+   if (fl[0] & 1) throw StatusError{MGPU_E_BIN_RANGE, "Co-ordinate out of range in BIN (fill_cells)"};
+   if (fl[0] & 2) throw StatusError{MGPU_E_BIN_ROUND, "Rounding problem in BIN (fill_cells)"};
+}
+
+// Real-space pass on the sites in e->d_site; forces accumulate into
+// e->d_force_site.  Returns pe (incl. -eintra) and the 6 virial terms.
+void real_pass(mgpu_engine *e, double *pe, double vir6[6], bool pre_virial, mgpu_fc_info *info) {
+   HIPCHK(hipMemsetAsync(e->d_iflags.p, 0, sizeof(int) * 40, e->stream));
+   run_binning(e, false);
+   double v7[7];
+   run_realspace(e, v7);
+   int tc[33];
+   check_flags(e, tc);
+   if (e->max_id > MGPU_MAX_ID) throw StatusError{MGPU_E_CAPACITY, "max_id > 16"};
+   *pe += v7[0] - e->eintra;
+   for (int q = 0; q < 6; q++) vir6[q] = v7[1 + q];
+   (void)pre_virial;
+   mgpu_fc_info fi{};
+   fi.nx = e->nx; fi.ny = e->ny; fi.nz = e->nz; fi.n_nabors = e->n_half;
+   fi.n_too_close = tc[0];
+   for (int q = 0; q < 16 && q < tc[0]; q++) { fi.too_close_i[q] = tc[1 + 2 * q]; fi.too_close_j[q] = tc[2 + 2 * q]; }
+   fi.eintra = e->eintra;
+   e->last_fc = fi;
+   if (info) *info = fi;
+}
+
+void recip_pass(mgpu_engine *e, double *pe, double st6[6], mgpu_ew_info *info) {
+   double o7[7];
+   if (e->precision == MGPU_MIXED) run_ewald_mixed(e, o7);
+   else run_ewald_fp64(e, o7);
+   HIPCHK(hipStreamSynchronize(e->stream));
+   double vol = host_det(e->h);
+   *pe += o7[0] - e->self_energy + e->sheet_energy / vol;
+   for (int q = 0; q < 6; q++) st6[q] = o7[1 + q];
+   st6[0] += e->sheet_energy / vol;
+   st6[3] += e->sheet_energy / vol;
+   st6[5] += e->sheet_energy / vol;
+   mgpu_ew_info wi{};
+   wi.hmax = e->hmax; wi.kmax = e->kmax; wi.lmax = e->lmax; wi.nhkl = e->nk; wi.nsitesxf = e->nsitesxf;
+   wi.self_energy = e->self_energy; wi.sheet_energy = e->sheet_energy;
+   e->last_ew = wi;
+   if (info) *info = wi;
+}
+
+}  // namespace
+
+// Pre-existing site_force contributes sum r_i F_i to the reference's virial
+// (force.c:974-982 sums over the accumulated array).
+__global__ void k_pre_virial(int ns, int nsp, const double *__restrict__ s, const double *__restrict__ f,
+                             double *__restrict__ part) {
+   __shared__ double ws[4][6];
+   int i = blockIdx.x * 256 + threadIdx.x;
+   double v[6] = {0, 0, 0, 0, 0, 0};
+   if (i < ns) {
+      double x = s[i], y = s[nsp + i], z = s[2 * nsp + i];
+      double fx = f[i], fy = f[nsp + i], fz = f[2 * nsp + i];
+      v[0] = x * fx; v[1] = y * fx; v[2] = z * fx; v[3] = y * fy; v[4] = z * fy; v[5] = z * fz;
+   }
+   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   for (int q = 0; q < 6; q++) {
+      double t = wave_sum(v[q]);
+      if (lane == 0) ws[w][q] = t;
+   }
+   __syncthreads();
+   if (threadIdx.x < 6)
+      part[6 * (size_t)blockIdx.x + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+// =============================== C ABI ===================================
+extern "C" {
+
+int mgpu_create(mgpu_engine **eng, int device) {
+   if (!eng) return MGPU_E_ARG;
+   *eng = nullptr;
+   mgpu_engine *e = new mgpu_engine();
+   e->device = device;
+   int rc = guarded(e, [&] {
+      HIPCHK(hipSetDevice(device));
+      HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+      e->d_iflags.reserve(64);
+      HIPCHK(hipMemset(e->d_iflags.p, 0, 64 * sizeof(int)));
+      return 0;
+   });
+   if (rc) {
+      delete e;
+      return rc;
+   }
+   *eng = e;
+   return 0;
+}
+
+void mgpu_destroy(mgpu_engine *e) {
+   if (!e) return;
+   (void)hipSetDevice(e->device);
+   if (e->stream) (void)hipStreamSynchronize(e->stream);
+   auto fr = [](auto &b) { b.free(); };
+   fr(e->d_site_type); fr(e->d_site_mol); fr(e->d_site_fw); fr(e->d_chg_site); fr(e->d_entry_isite);
+   fr(e->d_entry_num); fr(e->d_entry_ftype); fr(e->d_entry_mol); fr(e->d_potpar); fr(e->d_mol_first);
+   fr(e->d_mol_spec); fr(e->d_mol_quat); fr(e->d_pf); fr(e->d_spec_off); fr(e->d_spec_nsites);
+   fr(e->d_spec_rdof); fr(e->d_spec_fw); fr(e->d_site); fr(e->d_force_site); fr(e->d_cm); fr(e->d_quat);
+   fr(e->d_mforce); fr(e->d_mtorque); fr(e->d_red); fr(e->d_out); fr(e->d_iflags); fr(e->d_entry_cell);
+   fr(e->d_entry_slot); fr(e->d_cell_count); fr(e->d_cell_start); fr(e->d_cell_entries);
+   fr(e->d_cell_nsites); fr(e->d_cell_site_start); fr(e->d_scan_tmp); fr(e->d_corig); fr(e->d_cpos);
+   fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
+   fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
+   fr(e->d_groups); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   if (e->stream) (void)hipStreamDestroy(e->stream);
+   delete e;
+}
+
+const char *mgpu_last_error(const mgpu_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int mgpu_set_precision(mgpu_engine *e, int p) {
+   if (!e || (p != MGPU_FP64 && p != MGPU_MIXED)) return MGPU_E_ARG;
+   e->precision = p;
+   return 0;
+}
+
+int mgpu_set_system(mgpu_engine *e, const mgpu_system *s) {
+   return guarded(e, [&] {
+      HIPCHK(hipSetDevice(e->device));
+      if (!s || s->nsites <= 0 || s->nmols <= 0 || s->nspecies <= 0 || s->max_id <= 0)
+         return fail(e, MGPU_E_ARG, "invalid system description");
+      if (s->max_id > MGPU_MAX_ID) return fail(e, MGPU_E_CAPACITY, "more than 15 site types");
+      if (s->ptype < 0 || s->ptype > 6 || s->ptype == 5) return fail(e, MGPU_E_PTYPE, "unknown potential type");
+      e->nsites = s->nsites; e->nmols = s->nmols; e->nspecies = s->nspecies; e->max_id = s->max_id;
+      e->ptype = s->ptype; e->npar = s->n_potpar;
+      e->cutoff = s->cutoff; e->subcell = s->subcell; e->alpha = s->alpha; e->k_cutoff = s->k_cutoff;
+      e->strict = s->strict_cutoff; e->molpbc = s->molpbc; e->surface_dipole = s->surface_dipole;
+      e->spec_nmols.assign(s->spec_nmols, s->spec_nmols + s->nspecies);
+      e->spec_nsites.assign(s->spec_nsites, s->spec_nsites + s->nspecies);
+      e->spec_fw.assign(s->spec_framework, s->spec_framework + s->nspecies);
+      e->spec_rdof.assign(s->spec_rdof, s->spec_rdof + s->nspecies);
+      e->spec_off.assign(s->nspecies + 1, 0);
+      for (int i = 0; i < s->nspecies; i++) e->spec_off[i + 1] = e->spec_off[i] + s->spec_nsites[i];
+      int tot_spec_sites = e->spec_off[s->nspecies];
+      e->spec_site_id.assign(s->spec_site_id, s->spec_site_id + tot_spec_sites);
+      e->spec_pf.assign(s->spec_p_f_sites, s->spec_p_f_sites + 3 * tot_spec_sites);
+      e->site_charge.assign(s->site_charge, s->site_charge + s->max_id);
+      e->potpar.assign(s->potpar, s->potpar + (size_t)MGPU_NPOTP * s->max_id * s->max_id);
+      int nfw = 0;
+      for (int i = 0; i < s->nspecies; i++) nfw += s->spec_framework[i] ? s->spec_nmols[i] : 0;
+      if (nfw > 1) return fail(e, MGPU_E_MULTIFW, "Multiple framework molecules are not supported");
+      e->n_frame_types = nfw ? 2 : 1;
+      // per site / molecule / entry tables
+      e->site_type.clear(); e->site_mol.clear(); e->site_fw.clear();
+      e->entry_isite.clear(); e->entry_num.clear(); e->entry_ftype.clear(); e->entry_mol.clear();
+      std::vector<int> mol_first, mol_spec, mol_quat;
+      std::vector<double> chg;
+      int isite = 0, imol = 0, iq = 0;
+      e->nsitesxf = 0; e->nmolsxf = 0; e->fw_spec = -1;
+      for (int sp = 0; sp < s->nspecies; sp++) {
+         int ns = s->spec_nsites[sp];
+         bool fw = s->spec_framework[sp];
+         if (!fw && e->fw_spec < 0) { e->nsitesxf += ns * s->spec_nmols[sp]; e->nmolsxf += s->spec_nmols[sp]; }
+         if (fw && e->fw_spec < 0) e->fw_spec = sp;
+         for (int m = 0; m < s->spec_nmols[sp]; m++, imol++) {
+            mol_first.push_back(isite);
+            mol_spec.push_back(sp);
+            mol_quat.push_back(s->spec_rdof[sp] > 0 ? iq++ : -1);
+            if (!s->molpbc || fw) {
+               for (int k = 0; k < ns; k++) {
+                  e->entry_isite.push_back(isite + k); e->entry_num.push_back(1);
+                  e->entry_ftype.push_back(fw ? 1 : 0); e->entry_mol.push_back(-1);
+               }
+            } else {
+               e->entry_isite.push_back(isite); e->entry_num.push_back(ns);
+               e->entry_ftype.push_back(0); e->entry_mol.push_back(imol);
+            }
+            for (int k = 0; k < ns; k++, isite++) {
+               int id = s->spec_site_id[e->spec_off[sp] + k];
+               if (id < 0 || id >= s->max_id) return fail(e, MGPU_E_ARG, "site id out of range");
+               e->site_type.push_back(id);
+               e->site_mol.push_back(imol);
+               e->site_fw.push_back(fw ? 1 : 0);
+               chg.push_back(s->site_charge[id]);
+            }
+         }
+      }
+      if (isite != s->nsites || imol != s->nmols) return fail(e, MGPU_E_ARG, "site/molecule counts inconsistent");
+      e->nmols_r = iq;
+      e->nentries = (int)e->entry_isite.size();
+      e->nsp = ((s->nsites + 255) / 256) * 256;
+      auto up = [&](auto &buf, const auto &vec) {
+         buf.reserve(vec.size() + 1);
+         HIPCHK(hipMemcpy(buf.p, vec.data(), sizeof(vec[0]) * vec.size(), hipMemcpyHostToDevice));
+      };
+      up(e->d_site_type, e->site_type); up(e->d_site_mol, e->site_mol); up(e->d_site_fw, e->site_fw);
+      up(e->d_chg_site, chg); up(e->d_entry_isite, e->entry_isite); up(e->d_entry_num, e->entry_num);
+      up(e->d_entry_ftype, e->entry_ftype); up(e->d_entry_mol, e->entry_mol); up(e->d_potpar, e->potpar);
+      up(e->d_mol_first, mol_first); up(e->d_mol_spec, mol_spec); up(e->d_mol_quat, mol_quat);
+      up(e->d_pf, e->spec_pf); up(e->d_spec_off, e->spec_off); up(e->d_spec_nsites, e->spec_nsites);
+      up(e->d_spec_rdof, e->spec_rdof); up(e->d_spec_fw, e->spec_fw);
+      e->d_site.reserve(3 * (size_t)e->nsp);
+      e->d_force_site.reserve(3 * (size_t)e->nsp);
+      HIPCHK(hipMemset(e->d_site.p, 0, sizeof(double) * 3 * e->nsp));
+      e->d_cm.reserve(3 * (size_t)e->nmols + 3);
+      e->d_quat.reserve(4 * (size_t)std::max(e->nmols_r, 1));
+      e->d_mforce.reserve(3 * (size_t)e->nmols + 3);
+      e->d_mtorque.reserve(3 * (size_t)std::max(e->nmols_r, 1));
+      compute_statics(e, chg.data());
+      e->n_full = 0;
+      e->nk = 0;
+      e->have_system = true;
+      return 0;
+   });
+}
+
+int mgpu_force_calc(mgpu_engine *e, const double h[9], const double *site, int ld, const double *cm,
+                    const double *chg, double *site_force, double *pe, double stress[9], mgpu_fc_info *info) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      if (!site || !site_force || !pe || !stress || ld < e->nsites) return fail(e, MGPU_E_ARG, "bad buffers");
+      HIPCHK(hipSetDevice(e->device));
+      set_geometry(e, h);
+      upload_sites(e, site, ld);
+      upload_forces(e, site_force, ld);
+      if (e->molpbc) {
+         if (!cm) return fail(e, MGPU_E_ARG, "c_of_m required with molecular cutoff");
+         HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
+      }
+      if (chg) {
+         HIPCHK(hipMemcpyAsync(e->d_chg_site.p, chg, sizeof(double) * e->nsites, hipMemcpyHostToDevice, e->stream));
+      }
+      // pre-existing forces' share of the virial
+      int nb = (e->nsites + 255) / 256;
+      e->d_red.reserve((size_t)nb * 6 + 16);
+      e->d_out.reserve(64);
+      k_pre_virial<<<nb, 256, 0, e->stream>>>(e->nsites, e->nsp, e->d_site.p, e->d_force_site.p, e->d_red.p);
+      double pv[6];
+      sum_cols(e, e->d_red.p, nb, 6, pv);
+      double v6[6], pe_loc = 0.0;
+      real_pass(e, &pe_loc, v6, true, info);
+      download_forces(e, site_force, ld);
+      HIPCHK(hipStreamSynchronize(e->stream));
+      *pe += pe_loc;
+      const int up[6] = {0, 1, 2, 4, 5, 8};
+      for (int q = 0; q < 6; q++) stress[up[q]] += v6[q] + pv[q];
+      return 0;
+   });
+}
+
+int mgpu_ewald(mgpu_engine *e, const double h[9], const double *site, int ld, const double *chg,
+               double *site_force, double *pe, double stress[9], mgpu_ew_info *info) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      if (!site || !site_force || !pe || !stress || ld < e->nsites) return fail(e, MGPU_E_ARG, "bad buffers");
+      if (!(e->alpha > 0)) return fail(e, MGPU_E_STATE, "alpha <= 0: no reciprocal-space sum");
+      HIPCHK(hipSetDevice(e->device));
+      set_geometry(e, h);
+      upload_sites(e, site, ld);
+      upload_forces(e, site_force, ld);
+      if (chg)
+         HIPCHK(hipMemcpyAsync(e->d_chg_site.p, chg, sizeof(double) * e->nsites, hipMemcpyHostToDevice, e->stream));
+      double st6[6], pe_loc = 0.0;
+      recip_pass(e, &pe_loc, st6, info);
+      download_forces(e, site_force, ld);
+      HIPCHK(hipStreamSynchronize(e->stream));
+      *pe += pe_loc;
+      const int up[6] = {0, 1, 2, 4, 5, 8};
+      for (int q = 0; q < 6; q++) stress[up[q]] += st6[q];
+      return 0;
+   });
+}
+
+int mgpu_upload_state(mgpu_engine *e, const double h[9], const double *cm, const double *quat) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      HIPCHK(hipSetDevice(e->device));
+      set_geometry(e, h);
+      HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
+      if (e->nmols_r)
+         HIPCHK(hipMemcpyAsync(e->d_quat.p, quat, sizeof(double) * 4 * e->nmols_r, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      return 0;
+   });
+}
+
+int mgpu_eval_forces(mgpu_engine *e, double pe[2], double dip[3], double stress[9]) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      HIPCHK(hipSetDevice(e->device));
+      HIPCHK(hipMemsetAsync(e->d_force_site.p, 0, sizeof(double) * 3 * e->nsp, e->stream));
+      launch_make_sites(e, e->molpbc ? 1 : 0);
+      double per = 0.0, pek = 0.0, v6[6] = {0}, st6[6] = {0};
+      real_pass(e, &per, v6, false, nullptr);
+      if (e->alpha > 1e-7) recip_pass(e, &pek, st6, nullptr);
+      double vol = host_det(e->h);
+      run_glue(e, pe, dip, stress, pek, v6, st6);
+      pe[0] = per + e->dist / vol;
+      return 0;
+   });
+}
+
+int mgpu_download_forces(mgpu_engine *e, double *force, double *torque, double *site_xyz, double *site_force) {
+   return guarded(e, [&] {
+      HIPCHK(hipSetDevice(e->device));
+      if (force)
+         HIPCHK(hipMemcpyAsync(force, e->d_mforce.p, sizeof(double) * 3 * e->nmols, hipMemcpyDeviceToHost, e->stream));
+      if (torque && e->nmols_r)
+         HIPCHK(hipMemcpyAsync(torque, e->d_mtorque.p, sizeof(double) * 3 * e->nmols_r, hipMemcpyDeviceToHost,
+                               e->stream));
+      if (site_xyz)
+         for (int c = 0; c < 3; c++)
+            HIPCHK(hipMemcpyAsync(site_xyz + (size_t)c * e->nsites, e->d_site.p + (size_t)c * e->nsp,
+                                  sizeof(double) * e->nsites, hipMemcpyDeviceToHost, e->stream));
+      if (site_force) download_forces(e, site_force, e->nsites);
+      HIPCHK(hipStreamSynchronize(e->stream));
+      return 0;
+   });
+}
+
+int mgpu_last_info(const mgpu_engine *e, mgpu_fc_info *fc, mgpu_ew_info *ew) {
+   if (!e) return MGPU_E_ARG;
+   if (fc) *fc = e->last_fc;
+   if (ew) *ew = e->last_ew;
+   return 0;
+}
+
+int mgpu_cells(mgpu_engine *e, const double h[9], const double *site, int ld, const double *cm, int *dims3,
+               int *entry_cell, int *cell_start, int *cell_entries) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      HIPCHK(hipSetDevice(e->device));
+      set_geometry(e, h);
+      upload_sites(e, site, ld);
+      if (e->molpbc && cm)
+         HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemsetAsync(e->d_iflags.p, 0, sizeof(int) * 40, e->stream));
+      run_binning(e, false);
+      if (dims3) { dims3[0] = e->nx; dims3[1] = e->ny; dims3[2] = e->nz; }
+      if (entry_cell)
+         HIPCHK(hipMemcpyAsync(entry_cell, e->d_entry_cell.p, sizeof(int) * e->nentries, hipMemcpyDeviceToHost,
+                               e->stream));
+      if (cell_start)
+         HIPCHK(hipMemcpyAsync(cell_start, e->d_cell_start.p, sizeof(int) * (e->ncells + 1), hipMemcpyDeviceToHost,
+                               e->stream));
+      if (cell_entries)
+         HIPCHK(hipMemcpyAsync(cell_entries, e->d_cell_entries.p, sizeof(int) * e->nentries, hipMemcpyDeviceToHost,
+                               e->stream));
+      check_flags(e, nullptr);
+      return 0;
+   });
+}
+
+int mgpu_pairlist(mgpu_engine *e, const double h[9], const double *site, int ld, const double *cm, int strict,
+                  double cutoff, long *total, long *list_start, int *nab, int *pbctrans, int *nfnab) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      if (!total || !list_start) return fail(e, MGPU_E_ARG, "total and list_start required");
+      HIPCHK(hipSetDevice(e->device));
+      set_geometry(e, h);
+      upload_sites(e, site, ld);
+      if (e->molpbc && cm)
+         HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemsetAsync(e->d_iflags.p, 0, sizeof(int) * 40, e->stream));
+      run_binning(e, false);
+      check_flags(e, nullptr);
+      std::vector<int> off;
+      int n = neighbour_offsets(h, cutoff, e->nx, e->ny, e->nz, strict, off);
+      if (n < 0) return fail(e, -n, "cutoff radius exceeds the image-cell range (CUTOFF)");
+      run_pairlist_count(e, strict, cutoff, off.data(), n, total, list_start);
+      if (nab || pbctrans || nfnab) run_pairlist_fill(e, off.data(), n, *total, nab, pbctrans, nfnab);
+      return 0;
+   });
+}
+
+int mgpu_sync(mgpu_engine *e) {
+   return guarded(e, [&] {
+      HIPCHK(hipStreamSynchronize(e->stream));
+      return 0;
+   });
+}
+
+int mgpu_profile_enable(mgpu_engine *e, int on) {
+   if (!e) return MGPU_E_ARG;
+   e->prof_on = on != 0;
+   return 0;
+}
+int mgpu_profile_reset(mgpu_engine *e) {
+   if (!e) return MGPU_E_ARG;
+   e->prof.clear();
+   return 0;
+}
+int mgpu_profile_get(const mgpu_engine *e, const char *name, double *avg_ms, long *calls) {
+   if (!e || !name) return MGPU_E_ARG;
+   auto it = e->prof.find(name);
+   if (it == e->prof.end()) {
+      if (avg_ms) *avg_ms = 0;
+      if (calls) *calls = 0;
+      return 0;
+   }
+   if (avg_ms) *avg_ms = it->second.calls ? it->second.total_ms / it->second.calls : 0;
+   if (calls) *calls = it->second.calls;
+   return 0;
+}
+
+}  // extern "C"
+
+__global__ void k_sum_cols_e(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
+   __shared__ double ws[16];
+   int col = blockIdx.x;
+   double s = 0;
+   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
+   s = wave_sum(s);
+   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
+      out[col] = t;
+   }
+}
+
+void mgpu::sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *host_out) {
+   e->d_out.reserve(64);
+   k_sum_cols_e<<<ncols, 256, 0, e->stream>>>(in, nrows, ncols, e->d_out.p + 40);
+   HIPCHK(hipMemcpyAsync(host_out, e->d_out.p + 40, sizeof(double) * ncols, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+}

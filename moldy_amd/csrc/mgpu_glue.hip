@@ -1,0 +1,279 @@
+// eval_forces glue on the GPU (restates src/accel.c:398-617 around the
+// boundary calls): make_sites (src/algorith.c:169-217), the dipole and
+// surface-dipole term (accel.c:542-558), mol_force/mol_torque
+// (algorith.c:111-163) and the site-to-molecular virial correction
+// (accel.c:575-599).  Sums are per-block partials reduced in a fixed order.
+#include "mgpu_internal.h"
+
+using namespace mgpu;
+
+namespace {
+
+struct MsArgs {
+   double h[9], hinv[9];
+   int ns, nsp, mode;   // mode 0: wrap every site; 1: wrap none; 2: wrap framework only
+};
+
+#pragma clang fp contract(off)
+__global__ void k_make_sites(MsArgs a, const int *__restrict__ site_mol, const int *__restrict__ mol_first,
+                             const int *__restrict__ mol_spec, const int *__restrict__ mol_quat,
+                             const int *__restrict__ spec_off, const int *__restrict__ spec_fw,
+                             const double *__restrict__ pf, const double *__restrict__ cm,
+                             const double *__restrict__ quat, double *__restrict__ site) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.ns) return;
+   int m = site_mol[i], sp = mol_spec[m], iq = mol_quat[m];
+   const double *p = pf + 3 * (spec_off[sp] + (i - mol_first[m]));
+   const double *c = cm + 3 * m;
+   // centre of mass: mat_vec_mul(h, c_of_m_s)
+   double c0 = a.h[0] * c[0] + a.h[1] * c[1] + a.h[2] * c[2];
+   double c1 = a.h[3] * c[0] + a.h[4] * c[1] + a.h[5] * c[2];
+   double c2 = a.h[6] * c[0] + a.h[7] * c[1] + a.h[8] * c[2];
+   double x, y, z;
+   if (iq >= 0) {
+      // q_to_rot, src/quaterns.c:129-154, then mat_vec_mul
+      const double *qq = quat + 4 * iq;
+      double q0 = qq[0], q1 = qq[1], q2 = qq[2], q3 = qq[3];
+      double a01 = 2.0 * q0 * q1, a02 = 2.0 * q0 * q2, a03 = 2.0 * q0 * q3;
+      double a12 = 2.0 * q1 * q2, a13 = 2.0 * q1 * q3, a23 = 2.0 * q2 * q3;
+      double r01 = a12 - a03, r02 = a13 + a02, r10 = a12 + a03, r12 = a23 - a01;
+      double r20 = a13 - a02, r21 = a23 + a01;
+      q0 = q0 * q0; q1 = q1 * q1; q2 = q2 * q2; q3 = q3 * q3;
+      double r00 = q0 + q1 - q2 - q3, r11 = q0 - q1 + q2 - q3, r22 = q0 - q1 - q2 + q3;
+      double s0 = r00 * p[0] + r01 * p[1] + r02 * p[2];
+      double s1 = r10 * p[0] + r11 * p[1] + r12 * p[2];
+      double s2 = r20 * p[0] + r21 * p[1] + r22 * p[2];
+      x = s0 + c0; y = s1 + c1; z = s2 + c2;
+   } else {
+      x = p[0] + c0; y = p[1] + c1; z = p[2] + c2;
+   }
+   bool wrap = a.mode == 0 || (a.mode == 2 && spec_fw[sp]);
+   if (wrap) {
+      double tx = floor(a.hinv[0] * x + a.hinv[1] * y + a.hinv[2] * z + 0.5);
+      double ty = floor(a.hinv[3] * x + a.hinv[4] * y + a.hinv[5] * z + 0.5);
+      double tz = floor(a.hinv[6] * x + a.hinv[7] * y + a.hinv[8] * z + 0.5);
+      x -= a.h[0] * tx + a.h[1] * ty + a.h[2] * tz;
+      y -= a.h[3] * tx + a.h[4] * ty + a.h[5] * tz;
+      z -= a.h[6] * tx + a.h[7] * ty + a.h[8] * tz;
+   }
+   site[i] = x;
+   site[a.nsp + i] = y;
+   site[2 * a.nsp + i] = z;
+}
+#pragma clang fp contract(on)
+
+// Block partial sums of NV per-item values.
+template <int NV>
+__device__ void block_partial(double (&v)[NV], double *out) {
+   __shared__ double ws[4][NV];
+   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+   for (int q = 0; q < NV; q++) {
+      double t = wave_sum(v[q]);
+      if (lane == 0) ws[w][q] = t;
+   }
+   __syncthreads();
+   if (threadIdx.x < NV)
+      out[(size_t)blockIdx.x * NV + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_dipole(int ns, int nsp, const double *__restrict__ site,
+                                                const double *__restrict__ chg, double *__restrict__ part) {
+   int i = blockIdx.x * 256 + threadIdx.x;
+   double v[3] = {0, 0, 0};
+   if (i < ns) {
+      double q = chg[i];
+      v[0] = site[i] * q; v[1] = site[nsp + i] * q; v[2] = site[2 * nsp + i] * q;
+   }
+   block_partial<3>(v, part);
+}
+
+__global__ void k_surface(int ns, int nsp, double c0, double c1, double c2, const double *__restrict__ chg,
+                          double *__restrict__ f) {
+   int i = blockIdx.x * 256 + threadIdx.x;
+   if (i >= ns) return;
+   double q = chg[i];
+   f[i] -= c0 * q;
+   f[nsp + i] -= c1 * q;
+   f[2 * nsp + i] -= c2 * q;
+}
+
+struct MfArgs {
+   int nmols, nsp;
+};
+
+// mol_force and mol_torque for one molecule per thread.
+__global__ void k_mol_ft(MfArgs a, const int *__restrict__ mol_first, const int *__restrict__ mol_spec,
+                         const int *__restrict__ mol_quat, const int *__restrict__ spec_off,
+                         const int *__restrict__ spec_ns, const double *__restrict__ pf,
+                         const double *__restrict__ quat, const double *__restrict__ f,
+                         double *__restrict__ mforce, double *__restrict__ mtorque) {
+   int m = blockIdx.x * blockDim.x + threadIdx.x;
+   if (m >= a.nmols) return;
+   int sp = mol_spec[m], i0 = mol_first[m], n = spec_ns[sp], iq = mol_quat[m];
+   double F0 = 0, F1 = 0, F2 = 0;
+   for (int s = 0; s < n; s++) F0 += f[i0 + s];
+   for (int s = 0; s < n; s++) F1 += f[a.nsp + i0 + s];
+   for (int s = 0; s < n; s++) F2 += f[2 * a.nsp + i0 + s];
+   mforce[3 * m] = F0; mforce[3 * m + 1] = F1; mforce[3 * m + 2] = F2;
+   if (iq < 0) return;
+   const double *qq = quat + 4 * iq;
+   double q0 = qq[0], q1 = qq[1], q2 = qq[2], q3 = qq[3];
+   double a01 = 2.0 * q0 * q1, a02 = 2.0 * q0 * q2, a03 = 2.0 * q0 * q3;
+   double a12 = 2.0 * q1 * q2, a13 = 2.0 * q1 * q3, a23 = 2.0 * q2 * q3;
+   double r[9];
+   r[1] = a12 - a03; r[2] = a13 + a02; r[3] = a12 + a03; r[5] = a23 - a01; r[6] = a13 - a02;
+   r[7] = a23 + a01;
+   q0 = q0 * q0; q1 = q1 * q1; q2 = q2 * q2; q3 = q3 * q3;
+   r[0] = q0 + q1 - q2 - q3; r[4] = q0 - q1 + q2 - q3; r[8] = q0 - q1 - q2 + q3;
+   const double *p = pf + 3 * spec_off[sp];
+   double t[3] = {0, 0, 0};
+   for (int s = 0; s < n; s++) {
+      double fs0 = f[i0 + s], fs1 = f[a.nsp + i0 + s], fs2 = f[2 * a.nsp + i0 + s];
+      double pf0 = r[0] * fs0 + r[3] * fs1 + r[6] * fs2;   // inverse rotation
+      double pf1 = r[1] * fs0 + r[4] * fs1 + r[7] * fs2;
+      double pf2 = r[2] * fs0 + r[5] * fs1 + r[8] * fs2;
+      const double *ps = p + 3 * s;
+      t[0] += ps[1] * pf2 - ps[2] * pf1;
+      t[1] += ps[2] * pf0 - ps[0] * pf2;
+      t[2] += ps[0] * pf1 - ps[1] * pf0;
+   }
+   mtorque[3 * iq] = t[0]; mtorque[3 * iq + 1] = t[1]; mtorque[3 * iq + 2] = t[2];
+}
+
+// Per-site terms of the virial correction: non-framework sites contribute
+// f_i s_j, framework sites f_i p_j (principal-frame coordinates).
+__global__ __launch_bounds__(256) void k_vir_sites(int ns, int nsp, int nsxf, const int *__restrict__ site_mol,
+                                                   const int *__restrict__ mol_first,
+                                                   const int *__restrict__ mol_spec,
+                                                   const int *__restrict__ spec_off,
+                                                   const double *__restrict__ pf,
+                                                   const double *__restrict__ site,
+                                                   const double *__restrict__ f, double *__restrict__ part) {
+   int i = blockIdx.x * 256 + threadIdx.x;
+   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+   if (i < ns) {
+      double fa[3] = {f[i], f[nsp + i], f[2 * nsp + i]};
+      double sa[3];
+      if (i < nsxf) {
+         sa[0] = site[i]; sa[1] = site[nsp + i]; sa[2] = site[2 * nsp + i];
+      } else {
+         int m = site_mol[i];
+         const double *p = pf + 3 * (spec_off[mol_spec[m]] + (i - mol_first[m]));
+         sa[0] = p[0]; sa[1] = p[1]; sa[2] = p[2];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+         for (int b = 0; b < 3; b++) v[3 * a + b] = fa[a] * sa[b];
+   }
+   block_partial<9>(v, part);
+}
+
+// Molecular terms: F_i R_j over non-framework molecules, R = h c_of_m.
+__global__ __launch_bounds__(256) void k_vir_mols(int nmolsxf, double h0, double h1, double h2, double h3,
+                                                  double h4, double h5, double h6, double h7, double h8,
+                                                  const double *__restrict__ cm,
+                                                  const double *__restrict__ mforce,
+                                                  double *__restrict__ part) {
+   int m = blockIdx.x * 256 + threadIdx.x;
+   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+   if (m < nmolsxf) {
+      const double *c = cm + 3 * m;
+      double R[3] = {h0 * c[0] + h1 * c[1] + h2 * c[2], h3 * c[0] + h4 * c[1] + h5 * c[2],
+                     h6 * c[0] + h7 * c[1] + h8 * c[2]};
+      const double *F = mforce + 3 * m;
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+         for (int b = 0; b < 3; b++) v[3 * a + b] = F[a] * R[b];
+   }
+   block_partial<9>(v, part);
+}
+
+__global__ void k_sum_cols2(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
+   __shared__ double ws[16];
+   int col = blockIdx.x;
+   double s = 0;
+   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
+   s = wave_sum(s);
+   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
+      out[col] = t;
+   }
+}
+
+}  // namespace
+
+void mgpu::launch_make_sites(mgpu_engine *e, int mode) {
+   MsArgs a;
+   for (int i = 0; i < 9; i++) { a.h[i] = e->h[i]; a.hinv[i] = e->hinv[i]; }
+   a.ns = e->nsites; a.nsp = e->nsp; a.mode = mode;
+   ProfScope ps(e, "make_sites");
+   k_make_sites<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(
+       a, e->d_site_mol.p, e->d_mol_first.p, e->d_mol_spec.p, e->d_mol_quat.p, e->d_spec_off.p,
+       e->d_spec_fw.p, e->d_pf.p, e->d_cm.p, e->d_quat.p, e->d_site.p);
+}
+
+// Everything of eval_forces after the two boundary calls.  vir_real: the
+// upper-triangle real-space virial (6), st_recip: the Ewald stress (6).
+void mgpu::run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9], double pe_recip,
+                    const double *vir_real, const double *st_recip) {
+   ProfScope ps(e, "glue");
+   const int ns = e->nsites, nsp = e->nsp;
+   const int nb = (ns + 255) / 256, nmb = (e->nmols + 255) / 256;
+   const double PI_D = 3.14159265358979323846;
+   double vol = host_det(e->h);
+   e->d_red.reserve((size_t)(nb + nmb) * 9 + 64);
+   e->d_out.reserve(64);
+   double *out = e->d_out.p + 16;
+   // second make_sites: MOLPBC for molecules, SITEPBC for frameworks (accel.c:536-541)
+   launch_make_sites(e, 2);
+   double hd[3] = {0, 0, 0};
+   if (e->alpha > 1e-7) {
+      k_dipole<<<nb, 256, 0, e->stream>>>(ns, nsp, e->d_site.p, e->d_chg_site.p, e->d_red.p);
+      k_sum_cols2<<<3, 256, 0, e->stream>>>(e->d_red.p, nb, 3, out);
+      HIPCHK(hipMemcpyAsync(hd, out, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      if (e->surface_dipole) {
+         double c = 4.0 * PI_D / (3.0 * vol);
+         k_surface<<<nb, 256, 0, e->stream>>>(ns, nsp, c * hd[0], c * hd[1], c * hd[2], e->d_chg_site.p,
+                                              e->d_force_site.p);
+         pe_recip += 2.0 * PI_D / (3.0 * vol) * (hd[0] * hd[0] + hd[1] * hd[1] + hd[2] * hd[2]);
+      }
+   }
+   dip[0] = hd[0]; dip[1] = hd[1]; dip[2] = hd[2];
+   MfArgs mf{e->nmols, nsp};
+   k_mol_ft<<<nmb, 256, 0, e->stream>>>(mf, e->d_mol_first.p, e->d_mol_spec.p, e->d_mol_quat.p,
+                                        e->d_spec_off.p, e->d_spec_nsites.p, e->d_pf.p, e->d_quat.p,
+                                        e->d_force_site.p, e->d_mforce.p, e->d_mtorque.p);
+   k_vir_sites<<<nb, 256, 0, e->stream>>>(ns, nsp, e->nsitesxf, e->d_site_mol.p, e->d_mol_first.p,
+                                          e->d_mol_spec.p, e->d_spec_off.p, e->d_pf.p, e->d_site.p,
+                                          e->d_force_site.p, e->d_red.p);
+   int nmxb = (e->nmolsxf + 255) / 256;
+   if (nmxb == 0) nmxb = 1;
+   double *mpart = e->d_red.p + (size_t)nb * 9;
+   k_vir_mols<<<nmxb, 256, 0, e->stream>>>(e->nmolsxf, e->h[0], e->h[1], e->h[2], e->h[3], e->h[4],
+                                           e->h[5], e->h[6], e->h[7], e->h[8], e->d_cm.p, e->d_mforce.p,
+                                           mpart);
+   k_sum_cols2<<<9, 256, 0, e->stream>>>(e->d_red.p, nb, 9, out);
+   k_sum_cols2<<<9, 256, 0, e->stream>>>(mpart, nmxb, 9, out + 9);
+   double corr[18];
+   HIPCHK(hipMemcpyAsync(corr, out, 18 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   // assemble as accel.c:576-606
+   double st[9] = {0};
+   const int up[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+   for (int q = 0; q < 6; q++) st[3 * up[q][0] + up[q][1]] = vir_real[q] + st_recip[q];
+   for (int i = 0; i < 3; i++) {
+      for (int j = i + 1; j < 3; j++) st[3 * j + i] = st[3 * i + j];
+      for (int j = 0; j < 3; j++) st[3 * i + j] -= corr[3 * i + j] - corr[9 + 3 * i + j];
+   }
+   for (int i = 0; i < 3; i++) st[4 * i] += e->distp / vol;
+   for (int i = 0; i < 9; i++) stress[i] = st[i];
+   pe[1] = pe_recip;
+}

@@ -1,0 +1,216 @@
+// Internal definitions shared by the engine's HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+
+#include "../../include/moldy_amd.h"
+
+#define MGPU_NPOTP 8
+#define MGPU_MAX_ID 16          // site ids 0..15 (potential table in LDS)
+#define MGPU_IB 4               // i-sites register-blocked per wave
+#define MGPU_WAVE 64
+
+namespace mgpu {
+
+struct HipError {
+   hipError_t err;
+   const char *what;
+   int line;
+};
+
+#define HIPCHK(x)                                                   \
+   do {                                                             \
+      hipError_t e_ = (x);                                          \
+      if (e_ != hipSuccess) throw mgpu::HipError{e_, #x, __LINE__}; \
+   } while (0)
+
+struct StatusError {
+   int code;
+   std::string msg;
+};
+
+// Device buffer owned by the engine (grows, never shrinks).
+template <class T> struct DBuf {
+   T *p = nullptr;
+   size_t n = 0;
+   void reserve(size_t m) {
+      if (m <= n) return;
+      if (p) HIPCHK(hipFree(p));
+      p = nullptr;
+      size_t bytes = (m ? m : 1) * sizeof(T);
+      HIPCHK(hipMalloc((void **)&p, bytes));
+      n = m;
+   }
+   void free() {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+   }
+};
+
+// Per-kernel event timing (HIP events on the engine stream).
+struct KProf {
+   double total_ms = 0;
+   long calls = 0;
+};
+
+// Row of the reciprocal-lattice set: fixed (h,k), a contiguous run of l.
+struct KRow {
+   int h, k, l0, nl;     // l runs l0 .. l0+nl-1
+   int kstart;           // index of (h,k,l0) in the reference's k order
+   int pad;
+   double kx, ky, kzt;   // kx, ky and kz at l = 0 (src/ewald.c:445-447)
+};
+
+}  // namespace mgpu
+
+struct mgpu_engine {
+   int device = 0;
+   hipStream_t stream = nullptr;
+   int precision = MGPU_FP64;
+   std::string err;
+
+   // ---- system (host copies) ----
+   bool have_system = false;
+   int nsites = 0, nmols = 0, nmols_r = 0, nspecies = 0, max_id = 0, ptype = 0, npar = 0;
+   std::vector<int> spec_nmols, spec_nsites, spec_fw, spec_rdof, spec_site_id, spec_off;
+   std::vector<double> spec_pf, site_charge, potpar;
+   double cutoff = 0, subcell = 0, alpha = -1, k_cutoff = 0;
+   int strict = 0, molpbc = 0, surface_dipole = 0;
+   int nsitesxf = 0, nmolsxf = 0, fw_spec = -1;
+   std::vector<int> site_type, site_mol, site_fw;      // per site
+   std::vector<int> entry_isite, entry_num, entry_ftype, entry_mol;  // binning entries
+   int nentries = 0, n_frame_types = 1;
+   double eintra = 0, self_energy = 0, sheet_energy = 0, dist = 0, distp = 0;
+   bool statics_done = false;
+   int nsp = 0;        // padded site stride
+
+   // ---- static device data ----
+   mgpu::DBuf<int> d_site_type, d_site_mol, d_site_fw;
+   mgpu::DBuf<double> d_chg_site;             // per site, FP64
+   mgpu::DBuf<int> d_entry_isite, d_entry_num, d_entry_ftype, d_entry_mol;
+   mgpu::DBuf<double> d_potpar;                // [max_id*max_id*8]
+   mgpu::DBuf<int> d_mol_first, d_mol_spec, d_mol_quat;
+   mgpu::DBuf<double> d_pf;                    // concatenated p_f_sites
+   mgpu::DBuf<int> d_spec_off, d_spec_nsites, d_spec_rdof, d_spec_fw;
+
+   // ---- dynamic device data ----
+   double h[9] = {0}, hinv[9] = {0};
+   mgpu::DBuf<double> d_site;                  // [3][nsp]
+   mgpu::DBuf<double> d_force_site;            // [3][nsp]
+   mgpu::DBuf<double> d_cm;                    // [nmols][3] scaled
+   mgpu::DBuf<double> d_quat;                  // [nmols_r][4]
+   mgpu::DBuf<double> d_mforce, d_mtorque;     // [nmols][3], [nmols_r][3]
+   mgpu::DBuf<double> d_red;                   // reduction scratch
+   mgpu::DBuf<double> d_out;                   // small result block
+   mgpu::DBuf<int> d_iflags;                   // status flags / counters
+
+   // ---- cells ----
+   int nx = 0, ny = 0, nz = 0, ncells = 0;
+   mgpu::DBuf<int> d_entry_cell, d_entry_slot, d_cell_count, d_cell_start, d_cell_entries;
+   mgpu::DBuf<int> d_cell_nsites, d_cell_site_start, d_scan_tmp;
+   mgpu::DBuf<int> d_corig;                    // cell-ordered -> site index
+   mgpu::DBuf<double4> d_cpos;                 // cell-ordered x,y,z,q (FP64)
+   mgpu::DBuf<float4> d_cposf;                 // cell-local x,y,z,q (FP32)
+   mgpu::DBuf<int2> d_cmeta;                   // type | fw<<8, mol
+   // neighbour offsets (half shell in reference order, then the mirror)
+   int n_half = 0, n_full = 0;
+   std::vector<int> off_half;                  // [n_half][3]
+   mgpu::DBuf<int4> d_off;                     // full shell: dx,dy,dz,0
+   mgpu::DBuf<float4> d_offD;                  // h (dx/nx,dy/ny,dz/nz) as float
+   mgpu::DBuf<int4> d_off_pl;                  // offsets of an explicit pair-list call
+   mgpu::DBuf<int> d_pl_count, d_pl_start;     // pair-list CSR counts / starts
+   double off_key[13] = {0};
+   mgpu::DBuf<double> d_cell_part;             // per-cell energy/virial partials
+
+   // ---- ewald ----
+   int hmax = 0, kmax = 0, lmax = 0, nk = 0, nrows = 0;
+   std::vector<mgpu::KRow> rows;
+   std::vector<int> hkl;                        // [nk][3]
+   double kset_key[10] = {0};
+   mgpu::DBuf<mgpu::KRow> d_rows;
+   mgpu::DBuf<int> d_krow;                      // row of each k
+   mgpu::DBuf<double2> d_tx, d_ty, d_tz;        // trig tables [n+1][nsp]
+   mgpu::DBuf<double> d_sk;                     // S(k) partial sums
+   mgpu::DBuf<int4> d_groups;                   // S(k) workgroup -> k rows
+   mgpu::DBuf<double> d_wk;                     // per-k weights
+   int sk_chunks = 0;
+
+   // ---- mixed-precision ewald ----
+   mgpu::DBuf<uint8_t> d_mix;                   // scratch for the MFMA path
+   size_t mix_bytes = 0;
+
+   // ---- diagnostics ----
+   mgpu_fc_info last_fc{};
+   mgpu_ew_info last_ew{};
+   bool prof_on = false;
+   std::map<std::string, mgpu::KProf> prof;
+};
+
+namespace mgpu {
+// launch helpers implemented per translation unit
+void launch_make_sites(mgpu_engine *e, int molflag_mode);
+void run_binning(mgpu_engine *e, bool site_only_cellorder);
+void run_realspace(mgpu_engine *e, double *pe_vir_out /*7*/);
+void run_pairlist_count(mgpu_engine *e, int strict, double cutoff, const int *offs, int noff,
+                        long *total, long *list_start_host);
+void run_pairlist_fill(mgpu_engine *e, const int *offs, int noff, long total, int *nab,
+                       int *pbctrans, int *nfnab);
+void run_ewald_fp64(mgpu_engine *e, double *pe_stress_out /*7*/);
+void run_ewald_mixed(mgpu_engine *e, double *pe_stress_out /*7*/);
+void run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9], double pe_recip,
+              const double *vir_real, const double *st_recip);
+void device_scan_int(mgpu_engine *e, const int *in, int *out, int n);  // exclusive, out[n]=total
+void sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *host_out);
+double host_det(const double a[9]);
+void host_invert(const double a[9], double b[9]);
+
+struct ProfScope {
+   mgpu_engine *e;
+   const char *name;
+   hipEvent_t a{}, b{};
+   ProfScope(mgpu_engine *e_, const char *n) : e(e_), name(n) {
+      if (e->prof_on) {
+         HIPCHK(hipEventCreate(&a));
+         HIPCHK(hipEventCreate(&b));
+         HIPCHK(hipEventRecord(a, e->stream));
+      }
+   }
+   ~ProfScope() {
+      if (e->prof_on) {
+         float ms = 0;
+         (void)hipEventRecord(b, e->stream);
+         (void)hipEventSynchronize(b);
+         (void)hipEventElapsedTime(&ms, a, b);
+         auto &p = e->prof[name];
+         p.total_ms += ms;
+         p.calls++;
+         (void)hipEventDestroy(a);
+         (void)hipEventDestroy(b);
+      }
+   }
+};
+}  // namespace mgpu
+
+// ---------------------------------------------------------- device helpers
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+   return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+   return v;
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+   for (int o = 1; o < 64; o <<= 1) {
+      int t = __shfl_up(v, o, 64);
+      if (lane >= o) v += t;
+   }
+   return v;
+}

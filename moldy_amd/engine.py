@@ -1,0 +1,246 @@
+"""ctypes front end of libmoldy_amd.so (the C ABI in include/moldy_amd.h).
+
+``ForceEngine`` mirrors the reference's operator interface for this path:
+``force_calc`` and ``ewald`` take site coordinates and accumulate into site
+forces, energy and the upper-triangle virial exactly like src/force.c:1108
+and src/ewald.c:280; ``eval_forces`` is the device-resident restatement of
+src/accel.c:398-617.  The library is required: importing this module on a
+machine where it was not built raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmoldy_amd.so")
+
+FP64, MIXED = 0, 1
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+
+ERRORS = {1: "ARG", 2: "HIP", 3: "BIN_RANGE", 4: "BIN_ROUND", 5: "CUTOFF", 6: "TONAB",
+          7: "MULTIFW", 8: "PTYPE", 9: "STATE", 10: "CAPACITY"}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mgpu error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class MgpuSystem(C.Structure):
+    _fields_ = [("nsites", C.c_int), ("nmols", C.c_int), ("nspecies", C.c_int),
+                ("max_id", C.c_int), ("ptype", C.c_int), ("n_potpar", C.c_int),
+                ("spec_nmols", _ip), ("spec_nsites", _ip), ("spec_framework", _ip),
+                ("spec_rdof", _ip), ("spec_site_id", _ip), ("spec_p_f_sites", _dp),
+                ("site_charge", _dp), ("potpar", _dp),
+                ("cutoff", C.c_double), ("subcell", C.c_double), ("alpha", C.c_double),
+                ("k_cutoff", C.c_double), ("strict_cutoff", C.c_int), ("molpbc", C.c_int),
+                ("surface_dipole", C.c_int)]
+
+
+class FcInfo(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("n_nabors", C.c_int),
+                ("n_too_close", C.c_int), ("too_close_i", C.c_int * 16),
+                ("too_close_j", C.c_int * 16), ("n_list_entries", C.c_long),
+                ("max_cell_sites", C.c_int), ("eintra", C.c_double)]
+
+
+class EwInfo(C.Structure):
+    _fields_ = [("hmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int), ("nhkl", C.c_int),
+                ("nsitesxf", C.c_int), ("self_energy", C.c_double),
+                ("sheet_energy", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is not built; run `python -m moldy_amd.build`")
+        L = C.CDLL(LIBPATH)
+        L.mgpu_last_error.restype = C.c_char_p
+        L.mgpu_last_error.argtypes = [C.c_void_p]
+        L.mgpu_destroy.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class ForceEngine:
+    """One engine (HIP stream + device buffers) bound to one system."""
+
+    def __init__(self, system, device=0, precision=FP64):
+        L = lib()
+        h = C.c_void_p()
+        self._chk(L.mgpu_create(C.byref(h), int(device)), h)
+        self.h_ = h
+        self.sys = system
+        self.f = system.flat()
+        f, c = self.f, system.control
+        self.cs = MgpuSystem(system.nsites, system.nmols, system.nspecies, system.max_id,
+                             system.ptype, system.n_potpar,
+                             _p(f["spec_nmols"], _ip), _p(f["spec_nsites"], _ip),
+                             _p(f["spec_framework"], _ip), _p(f["spec_rdof"], _ip),
+                             _p(f["spec_site_id"], _ip), _p(f["spec_p_f_sites"]),
+                             _p(f["site_charge"]), _p(f["potpar"]),
+                             c.cutoff, c.subcell, c.alpha, c.k_cutoff, int(c.strict_cutoff),
+                             int(c.molpbc), int(c.surface_dipole))
+        self._chk(L.mgpu_set_system(self.h_, C.byref(self.cs)))
+        self.set_precision(precision)
+        self.chg = system.site_charges()
+
+    def _chk(self, rc, h=None):
+        if rc:
+            hh = h if h is not None else self.h_
+            msg = lib().mgpu_last_error(hh).decode() if hh else ""
+            raise EngineError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h_", None):
+            lib().mgpu_destroy(self.h_)
+            self.h_ = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, p):
+        self._chk(lib().mgpu_set_precision(self.h_, int(p)))
+        self.precision = p
+
+    # ------------------------------------------------------- the boundary
+    def force_calc(self, h, xyz, fxyz=None, chg=None, c_of_m=None):
+        n = self.sys.nsites
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        f = np.zeros((3, n)) if fxyz is None else fxyz
+        pe = C.c_double(0.0)
+        st = np.zeros(9)
+        info = FcInfo()
+        h = np.ascontiguousarray(h, np.float64)
+        cm = np.ascontiguousarray(self.sys.c_of_m if c_of_m is None else c_of_m, np.float64)
+        q = np.ascontiguousarray(self.chg if chg is None else chg, np.float64)
+        self._chk(lib().mgpu_force_calc(self.h_, _p(h), _p(xyz), n, _p(cm), _p(q), _p(f),
+                                        C.byref(pe), _p(st), C.byref(info)))
+        return dict(force=f, pe=pe.value, stress=st.reshape(3, 3), info=info)
+
+    def ewald(self, h, xyz, fxyz=None, chg=None):
+        n = self.sys.nsites
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        f = np.zeros((3, n)) if fxyz is None else fxyz
+        pe = C.c_double(0.0)
+        st = np.zeros(9)
+        info = EwInfo()
+        h = np.ascontiguousarray(h, np.float64)
+        q = np.ascontiguousarray(self.chg if chg is None else chg, np.float64)
+        self._chk(lib().mgpu_ewald(self.h_, _p(h), _p(xyz), n, _p(q), _p(f), C.byref(pe),
+                                   _p(st), C.byref(info)))
+        return dict(force=f, pe=pe.value, stress=st.reshape(3, 3), info=info)
+
+    # --------------------------------------------------- device resident
+    def upload_state(self, h=None, c_of_m=None, quat=None):
+        s = self.sys
+        h = np.ascontiguousarray(s.h if h is None else h, np.float64)
+        cm = np.ascontiguousarray(s.c_of_m if c_of_m is None else c_of_m, np.float64)
+        q = np.ascontiguousarray(self.f["quat"] if quat is None else quat, np.float64)
+        self._chk(lib().mgpu_upload_state(self.h_, _p(h), _p(cm), _p(q)))
+
+    def eval_forces_device(self):
+        pe = np.zeros(2)
+        dip = np.zeros(3)
+        st = np.zeros(9)
+        self._chk(lib().mgpu_eval_forces(self.h_, _p(pe), _p(dip), _p(st)))
+        return pe, dip, st.reshape(3, 3)
+
+    def download(self):
+        s = self.sys
+        force = np.zeros((s.nmols, 3))
+        torque = np.zeros((max(s.nmols_r, 1), 3))
+        sx = np.zeros((3, s.nsites))
+        sf = np.zeros((3, s.nsites))
+        self._chk(lib().mgpu_download_forces(self.h_, _p(force), _p(torque), _p(sx), _p(sf)))
+        return force, torque[:s.nmols_r], sx, sf
+
+    def eval_forces(self, h=None, c_of_m=None, quat=None):
+        self.upload_state(h, c_of_m, quat)
+        pe, dip, st = self.eval_forces_device()
+        force, torque, sx, sf = self.download()
+        return dict(pe=pe, dip=dip, stress=st, force=force, torque=torque, site_xyz=sx,
+                    site_force=sf)
+
+    def last_info(self):
+        fc, ew = FcInfo(), EwInfo()
+        lib().mgpu_last_info(self.h_, C.byref(fc), C.byref(ew))
+        return fc, ew
+
+    # ------------------------------------------------ cells and pair lists
+    def cells(self, h, xyz, c_of_m=None):
+        s = self.sys
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        h = np.ascontiguousarray(h, np.float64)
+        cm = np.ascontiguousarray(s.c_of_m if c_of_m is None else c_of_m, np.float64)
+        dims = np.zeros(3, np.int32)
+        self._chk(lib().mgpu_cells(self.h_, _p(h), _p(xyz), s.nsites, _p(cm), _p(dims, _ip),
+                                   None, None, None))
+        nc = int(np.prod(dims))
+        ne = s.nsites if not s.control.molpbc else None
+        ne = self._nentries()
+        ec = np.zeros(ne, np.int32)
+        cs = np.zeros(nc + 1, np.int32)
+        ce = np.zeros(ne, np.int32)
+        self._chk(lib().mgpu_cells(self.h_, _p(h), _p(xyz), s.nsites, _p(cm), _p(dims, _ip),
+                                   _p(ec, _ip), _p(cs, _ip), _p(ce, _ip)))
+        return dict(dims=tuple(dims), entry_cell=ec, cell_start=cs, cell_entries=ce)
+
+    def _nentries(self):
+        s = self.sys
+        if not s.control.molpbc:
+            return s.nsites
+        return sum(sp.nmols * (sp.nsites if sp.framework else 1) for sp in s.species)
+
+    def pairlist(self, h, xyz, strict=None, cutoff=None, c_of_m=None):
+        s = self.sys
+        c = s.control
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        h = np.ascontiguousarray(h, np.float64)
+        cm = np.ascontiguousarray(s.c_of_m if c_of_m is None else c_of_m, np.float64)
+        strict = int(c.strict_cutoff if strict is None else strict)
+        cut = c.cutoff if cutoff is None else cutoff
+        dims = self.cells(h, xyz, cm)["dims"]
+        nc = int(np.prod(dims))
+        total = C.c_long()
+        starts = np.zeros(nc + 1, np.int64)
+        self._chk(lib().mgpu_pairlist(self.h_, _p(h), _p(xyz), s.nsites, _p(cm), strict,
+                                      C.c_double(cut), C.byref(total), _p(starts, _lp),
+                                      None, None, None))
+        n = total.value
+        nab = np.zeros(n + 1, np.int32)
+        pbc = np.zeros(n + 1, np.int32)
+        nf = np.zeros((nc, 2), np.int32)
+        self._chk(lib().mgpu_pairlist(self.h_, _p(h), _p(xyz), s.nsites, _p(cm), strict,
+                                      C.c_double(cut), C.byref(total), _p(starts, _lp),
+                                      _p(nab, _ip), _p(pbc, _ip), _p(nf, _ip)))
+        return dict(start=starts, nab=nab[:n], pbctrans=pbc[:n], nfnab=nf)
+
+    # --------------------------------------------------------- profiling
+    def profile(self, on=True):
+        lib().mgpu_profile_enable(self.h_, int(on))
+        lib().mgpu_profile_reset(self.h_)
+
+    def profile_get(self, name):
+        ms = C.c_double()
+        n = C.c_long()
+        lib().mgpu_profile_get(self.h_, name.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def sync(self):
+        self._chk(lib().mgpu_sync(self.h_))

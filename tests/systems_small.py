@@ -1,0 +1,74 @@
+"""Small systems for parity tests (seeded; the oracle finishes in seconds)."""
+import math
+
+import numpy as np
+
+from moldy_amd import system, units, sysspec
+import json
+import os
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "reference_examples.json")
+
+
+def water(n=216, seed=11, **kw):
+    kw.setdefault("cutoff", 6.5)
+    kw.setdefault("alpha", 0.40)
+    return system.water_box(n, seed=seed, **kw)
+
+
+def salt(seed=12):
+    return system.salt_water(nwater=400, nmg=8, ncl=16, seed=seed, cutoff=7.0, alpha=0.38)
+
+
+def argon(strict=True):
+    s = system.argon_fcc(ncell=4, jitter=0.05, cutoff=8.5)
+    s.control.strict_cutoff = strict
+    s.control.subcell = 2.0
+    return s
+
+
+def silica(seed=13):
+    return system.silica(ncell=2, seed=seed, cutoff=6.5, alpha=0.42)
+
+
+def example(name):
+    fx = json.load(open(GOLDEN))[name]
+    ctl = sysspec.parse_control(fx["control"])
+    sp = sysspec.parse_sysspec(fx["sysspec"])
+    s = sysspec.build_system(ctl, sp, name)
+    return s, fx["expected"]
+
+
+def two_species_generic(ptype, seed=21, n=150):
+    """Two charged monatomic species with parameters for potential ``ptype``
+    (generic, HIW, Morse, MCY ...), to exercise every kernel form."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    L = 16.0
+    pos = rng.random((2 * n, 3)) - 0.5
+    # keep sites apart: place on a jittered grid
+    g = int(math.ceil((2 * n) ** (1 / 3)))
+    grid = np.stack(np.meshgrid(*[np.arange(g)] * 3, indexing="ij"), -1).reshape(-1, 3)[:2 * n]
+    pos = (grid + 0.5) / g - 0.5 + rng.normal(0, 0.03, (2 * n, 3))
+    pos -= np.floor(pos + 0.5)
+    max_id = 3
+    q = np.zeros(max_id)
+    q[1], q[2] = units.charge_to_program(0.8), units.charge_to_program(-0.8)
+    pp = system.empty_potpar(max_id)
+    par = {
+        0: ([2.0, 1.9], [1.5, 2.0], [1.0, 2.1]),
+        1: ([300.0, 8.0e4, 3.5], [200.0, 6.0e4, 3.4], [150.0, 5e4, 3.3]),
+        2: ([8.0e4, 3.5, 200.0, 1.8], [6.0e4, 3.4, 150.0, 1.7], [5e4, 3.3, 100.0, 1.6]),
+        3: ([8.0e4, 3.5, 1.0e5, 50.0, 300.0, 400.0], [6.0e4, 3.4, 8e4, 40.0, 250.0, 300.0],
+            [5e4, 3.3, 7e4, 30.0, 200.0, 200.0]),
+        4: ([50.0, 300.0, 1.0e5], [40.0, 250.0, 8e4], [30.0, 200.0, 7e4]),
+        6: ([3.0e4, 0.5, 3.0, 200.0, 5.0, 1.5, 3.0], [2.5e4, 0.4, 3.1, 150.0, 4.0, 1.4, 3.1],
+            [2.0e4, 0.3, 3.2, 100.0, 3.0, 1.3, 3.2]),
+    }[ptype]
+    system.set_pair(pp, 1, 1, [x * 100.0 for x in par[0]] if ptype == 0 else par[0])
+    system.set_pair(pp, 1, 2, [x * 100.0 for x in par[1]] if ptype == 0 else par[1])
+    system.set_pair(pp, 2, 2, [x * 100.0 for x in par[2]] if ptype == 0 else par[2])
+    ctl = system.Control(cutoff=6.0, alpha=0.45, k_cutoff=2 * 0.45 * math.sqrt(-math.log(1e-6)))
+    sp = [system.Species("a", n, [1], np.zeros((1, 3))), system.Species("b", n, [2], np.zeros((1, 3)))]
+    return system.MoldySystem(sp, np.array([0, 20.0, 30.0]), q, pp, ptype, ctl,
+                              np.eye(3) * L, pos, np.zeros((0, 4)), f"pair{ptype}")

@@ -1,0 +1,194 @@
+"""GPU parity: the HIP engine against the CPU oracle on identical inputs.
+
+Bit-exact: cell assignment, cell list order, site neighbour lists.
+FP64 path: forces, energies and virial within 1e-10 relative (norm-wise).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+import systems_small as S
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+@pytest.fixture(scope="module")
+def eng_mod():
+    from moldy_amd import engine
+    return engine
+
+
+def _oracle(s):
+    from oracle.oracle import Oracle
+    return Oracle(s)
+
+
+CASES = {
+    "water": lambda: S.water(),
+    "water_strict": lambda: S.water(strict_cutoff=True) if False else _strict(S.water()),
+    "water_molpbc": lambda: _molpbc(S.water()),
+    "salt": S.salt,
+    "argon_strict": lambda: S.argon(True),
+    "argon": lambda: S.argon(False),
+    "silica": S.silica,
+    "clay": lambda: S.example("clay")[0],
+    "mgclh2o": lambda: _mgcl(),
+}
+
+
+def _strict(s):
+    s.control.strict_cutoff = True
+    return s
+
+
+def _molpbc(s):
+    s.control.molpbc = True
+    return s
+
+
+def _mgcl():
+    s, e = S.example("mgclh2o")
+    # positions from a seeded lattice (the logged run used a random start)
+    rng = np.random.Generator(np.random.Philox(3))
+    n = s.nmols
+    g = int(np.ceil(n ** (1 / 3)))
+    grid = np.stack(np.meshgrid(*[np.arange(g)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pick = rng.permutation(len(grid))[:n]
+    s.c_of_m = (grid[pick] + 0.5) / g - 0.5
+    from moldy_amd.system import random_quaternions
+    s.quat = random_quaternions(rng, s.nmols_r)
+    L = (np.prod(np.diag(s.h)) if s.h[0, 0] > 0 else 18.7676)
+    s.h = np.eye(3) * 18.76756976
+    return s
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_cells_bit_exact(eng_mod, case):
+    s = CASES[case]()
+    o = _oracle(s)
+    xyz = o.make_sites()
+    ref = o.fill_cells(xyz)
+    e = eng_mod.ForceEngine(s)
+    got = e.cells(s.h, xyz)
+    assert got["dims"] == ref["dims"]
+    assert np.array_equal(got["entry_cell"], ref["entry_cell"])
+    assert np.array_equal(got["cell_start"], ref["cell_start"])
+    assert np.array_equal(got["cell_entries"], ref["cell_entries"])
+    e.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_pairlist_bit_exact(eng_mod, case):
+    s = CASES[case]()
+    o = _oracle(s)
+    xyz = o.make_sites()
+    ref = o.site_lists(xyz)
+    e = eng_mod.ForceEngine(s)
+    got = e.pairlist(s.h, xyz)
+    assert np.array_equal(got["start"], ref["start"])
+    assert np.array_equal(got["nab"], ref["nab"])
+    assert np.array_equal(got["pbctrans"], ref["pbctrans"])
+    assert np.array_equal(got["nfnab"], ref["nfnab"])
+    e.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_force_calc_fp64(eng_mod, case):
+    s = CASES[case]()
+    o = _oracle(s)
+    xyz = o.make_sites()
+    ref = o.force_calc(xyz)
+    assert ref["rc"] == 0
+    e = eng_mod.ForceEngine(s)
+    got = e.force_calc(s.h, xyz)
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    assert abs(got["pe"] - ref["pe"]) <= TOL64 * max(abs(ref["pe"]), 1e-300)
+    assert rel_err(np.triu(got["stress"]), np.triu(ref["stress"])) < TOL64
+    assert got["info"].n_too_close == ref["info"].n_too_close
+    assert got["info"].nx == ref["info"].nx and got["info"].n_nabors == ref["info"].n_nabors
+    e.close()
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if not c.startswith("argon")])
+def test_ewald_fp64(eng_mod, case):
+    s = CASES[case]()
+    o = _oracle(s)
+    xyz = o.make_sites()
+    ref = o.ewald(xyz)
+    e = eng_mod.ForceEngine(s)
+    got = e.ewald(s.h, xyz)
+    assert got["info"].nhkl == ref["info"].nhkl
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    assert abs(got["pe"] - ref["pe"]) <= TOL64 * abs(ref["pe"])
+    assert rel_err(np.triu(got["stress"]), np.triu(ref["stress"])) < TOL64
+    e.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_eval_forces_fp64(eng_mod, case):
+    s = CASES[case]()
+    o = _oracle(s)
+    ref = o.eval_forces()
+    assert ref["rc"] == 0
+    e = eng_mod.ForceEngine(s)
+    got = e.eval_forces()
+    assert np.allclose(got["site_xyz"], ref["site_xyz"], rtol=0, atol=1e-12)
+    assert rel_err(got["site_force"], ref["site_force"]) < TOL64
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    if s.nmols_r:
+        assert rel_err(got["torque"], ref["torque"]) < TOL64
+    for k in range(2):
+        assert abs(got["pe"][k] - ref["pe"][k]) <= TOL64 * max(abs(ref["pe"][k]), 1e-300)
+    assert rel_err(got["stress"], ref["stress"]) < TOL64
+    assert rel_err(got["dip"], ref["dip"]) < TOL64 or np.linalg.norm(ref["dip"]) < 1e-9
+    e.close()
+
+
+@pytest.mark.parametrize("ptype", [0, 1, 2, 3, 4, 6])
+def test_every_potential_form(eng_mod, ptype):
+    s = S.two_species_generic(ptype)
+    o = _oracle(s)
+    xyz = o.make_sites()
+    ref = o.force_calc(xyz)
+    e = eng_mod.ForceEngine(s)
+    got = e.force_calc(s.h, xyz)
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    assert abs(got["pe"] - ref["pe"]) <= TOL64 * abs(ref["pe"])
+    # the same with no Ewald sum (kernel.c:356-461 forms)
+    s.control.alpha = -1.0
+    o = _oracle(s)
+    ref = o.force_calc(xyz)
+    e2 = eng_mod.ForceEngine(s)
+    got = e2.force_calc(s.h, xyz)
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    assert abs(got["pe"] - ref["pe"]) <= TOL64 * abs(ref["pe"])
+    e.close()
+    e2.close()
+
+
+def test_clay_static_energy_matches_moldy_log(eng_mod):
+    """The logged static clay run (step 0): real -5068.5, recip -4508.4 kJ/mol."""
+    from moldy_amd import units
+    s, exp = S.example("clay")
+    e = eng_mod.ForceEngine(s)
+    got = e.eval_forces()
+    assert abs(got["pe"][0] * units.CONV_E - exp["pe_real_kjmol"]) < 0.06
+    assert abs(got["pe"][1] * units.CONV_E - exp["pe_recip_kjmol"]) < 0.06
+    e.close()
+
+
+def test_force_calc_accumulates(eng_mod):
+    """Callee accumulates into site_force/pe/stress (src/force.c:1114-1115)."""
+    s = S.water(n=64)
+    o = _oracle(s)
+    xyz = o.make_sites()
+    rng = np.random.default_rng(0)
+    f0 = rng.normal(size=(3, s.nsites))
+    ref = o.force_calc(xyz, f0.copy())
+    e = eng_mod.ForceEngine(s)
+    got = e.force_calc(s.h, xyz, f0.copy())
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    assert rel_err(np.triu(got["stress"]), np.triu(ref["stress"])) < TOL64
+    e.close()
