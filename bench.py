@@ -1,0 +1,179 @@
+"""Throughput of one force evaluation (Moldy's eval_forces: make_sites,
+force_calc, ewald and the glue) on one MI355X, inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
+
+Workload: aqueous MgCl2, 32768 rigid 4-site waters + 600 Mg2+ + 1200 Cl-
+(132,872 sites, 1.0 g/cm^3, r_c = 10 A, alpha = 0.30/A, k_c = 2.2302/A),
+seeded synthetic configuration.  A step is one force evaluation; force-evals/s
+equals MD steps/s and ns/day assumes dt = 1 fs.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "force-evals/s + ns/day; mixed vs FP64 GPU at ≤1e-5 RMS rel force error"
+DT_FS = 1.0
+
+# Algorithmic work per (k-vector, site) in the reciprocal sum, counted on the
+# reference loops (src/ewald.c:126-131, 528-531, 558-566): 6 flops for the
+# q sin/cos addition step, 2 for the two sums, 9 for the force update.
+FLOPS_SK_PER_KI = 8.0
+FLOPS_FORCE_PER_KI = 9.0
+# Real-space flops per evaluated pair (mk_r_sqr 11 + kernel LJ+erfc 39 +
+# mk_forces 9), src/force.c:700-739, src/kernel.c:195-211.
+FLOPS_PER_PAIR = 59.0
+PEAK = {"fp64_vector": 78.6e12, "fp32_vector": 157.3e12, "fp16_mfma": 2.5e15}
+
+
+def build_workload(small=False):
+    from moldy_amd import system
+    if small:
+        return system.salt_water(nwater=4096, nmg=75, ncl=150, seed=3)
+    return system.salt_water(nwater=32768, nmg=600, ncl=1200, seed=3)
+
+
+def rms_rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def time_gpu(eng, steps, warmup):
+    import ctypes
+    from moldy_amd import engine as E
+    for _ in range(warmup):
+        eng.eval_forces_device()
+    eng.sync()
+    eng.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.eval_forces_device()
+    eng.sync()
+    t1 = time.perf_counter()
+    prof = {}
+    for name in ["binning", "realspace", "ewald_trig", "ewald_sfac", "ewald_kspace",
+                 "ewald_force", "ewald_mfma_sk", "ewald_mfma_force", "ewald_mfma_prep",
+                 "make_sites", "glue"]:
+        ms, n = eng.profile_get(name)
+        if n:
+            prof[name] = ms
+    eng.profile(False)
+    return (t1 - t0) / steps, prof
+
+
+def cpu_baseline(s, budget_s=25.0):
+    """Oracle (single core) on the same configuration: force_calc in full,
+    ewald over a k-space subset, extrapolated linearly in the k count."""
+    from oracle.oracle import Oracle
+    o = Oracle(s)
+    xyz = o.make_sites()
+    t0 = time.perf_counter()
+    o.force_calc(xyz)
+    t_real = time.perf_counter() - t0
+    nk_full = o.kset()[0].shape[0]
+    # pick a reduced k cutoff giving ~8% of the k vectors
+    s2 = s.copy()
+    frac = 0.08
+    s2.control.k_cutoff = s.control.k_cutoff * frac ** (1.0 / 3.0)
+    o2 = Oracle(s2)
+    nk_s = o2.kset()[0].shape[0]
+    t0 = time.perf_counter()
+    o2.ewald(xyz)
+    t_rec = time.perf_counter() - t0
+    t_total = t_real + t_rec * nk_full / max(nk_s, 1)
+    return dict(value=1.0 / t_total, unit="force-evals/s", cores=1, kind="port",
+                sample=(f"oracle/moldy_oracle.c on the same {s.nsites}-site configuration: "
+                        f"force_calc timed in full ({t_real:.2f} s); ewald timed over "
+                        f"{nk_s} of {nk_full} k-vectors ({t_rec:.2f} s) and scaled by "
+                        f"{nk_full / max(nk_s, 1):.1f}"),
+                seconds_per_eval=t_total)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="17k-site variant (smoke only)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the force "
+              "path is built for a single MI355X; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+    from moldy_amd import engine as E
+    s = build_workload(a.small)
+    print(f"[bench] workload {s.name}: {s.nsites} sites", file=sys.stderr, flush=True)
+    eng = E.ForceEngine(s, precision=E.FP64)
+    eng.upload_state()
+
+    # FP64 leg
+    t64, prof64 = time_gpu(eng, a.steps, a.warmup)
+    fc, ew = eng.last_info()
+    pe64, _, st64 = eng.eval_forces_device()
+    f64, tq64, _, sf64 = eng.download()
+    print(f"[bench] fp64 {1e3 * t64:.3f} ms/eval {prof64}", file=sys.stderr, flush=True)
+    nk, ns = ew.nhkl, s.nsites
+
+    res = dict(metric=METRIC, n_gpus=1, steps=a.steps, warmup=a.warmup,
+               higher_is_better=True, scaling="weak", vs_baseline=None, data="synthetic")
+    mixed = None
+    if a.precision == "mixed":
+        eng.set_precision(E.MIXED)
+        tmx, profmx = time_gpu(eng, a.steps, a.warmup)
+        pemx, _, stmx = eng.eval_forces_device()
+        fmx, tqmx, _, sfmx = eng.download()
+        print(f"[bench] mixed {1e3 * tmx:.3f} ms/eval {profmx}", file=sys.stderr, flush=True)
+        mixed = dict(t=tmx, prof=profmx,
+                     err=dict(site_force=rms_rel(sfmx, sf64), mol_force=rms_rel(fmx, f64),
+                              torque=rms_rel(tqmx, tq64) if s.nmols_r else 0.0,
+                              pe_real=abs(pemx[0] - pe64[0]) / abs(pe64[0]),
+                              pe_recip=abs(pemx[1] - pe64[1]) / abs(pe64[1]),
+                              stress=rms_rel(stmx, st64)))
+    t = mixed["t"] if mixed else t64
+    evals = 1.0 / t
+    res.update(value=evals, unit="force-evals/s", ms_per_step=1e3 * t,
+               ns_per_day=evals * DT_FS * 86400 / 1e6,
+               dtype="f32 pairs + f16x2 MFMA Ewald + f64 reductions" if mixed else "f64",
+               config=dict(workload=f"aqueous MgCl2: 32768 rigid 4-site water + 600 Mg2+ + 1200 Cl- "
+                                    f"({ns} sites)", sites=ns, k_vectors=nk, cutoff_A=10.0,
+                           alpha_per_A=0.30, k_cutoff_per_A=round(s.control.k_cutoff, 4),
+                           cells=[fc.nx, fc.ny, fc.nz], dt_fs=DT_FS,
+                           precision=a.precision))
+    res["fp64_evals_per_s"] = 1.0 / t64
+    res["fp64_ms_per_eval"] = 1e3 * t64
+    if mixed:
+        res["mixed_over_fp64"] = t64 / mixed["t"]
+        res["mixed_error_vs_fp64"] = mixed["err"]
+    prof = mixed["prof"] if mixed else prof64
+    # roofline of the dominant kernel
+    dom = max(prof, key=prof.get)
+    kind = {"ewald_sfac": ("mfma" if False else "fp64_vector", FLOPS_SK_PER_KI * nk * ns),
+            "ewald_force": ("fp64_vector", FLOPS_FORCE_PER_KI * nk * ns),
+            "ewald_mfma_sk": ("fp16_mfma", FLOPS_SK_PER_KI * nk * ns),
+            "ewald_mfma_force": ("fp16_mfma", FLOPS_FORCE_PER_KI * nk * ns)}.get(dom)
+    if kind:
+        peak = PEAK[kind[0]]
+        ach = kind[1] / (prof[dom] * 1e-3)
+        res["roofline"] = dict(kernel=dom, bound="mfma" if "mfma" in kind[0] else "valu",
+                               achieved=ach / 1e12, peak=peak / 1e12, unit="TFLOP/s",
+                               frac=ach / peak, traffic=None,
+                               flops_per_launch=kind[1], avg_ms=prof[dom])
+    res["kernel_ms"] = prof
+    if not a.no_cpu:
+        cb = cpu_baseline(s)
+        res["cpu_baseline"] = cb
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -238,6 +238,26 @@ void set_geometry(mgpu_engine *e, const double h[9]) {
       HIPCHK(hipMemcpyAsync(e->d_offD.p, D.data(), sizeof(float4) * D.size(), hipMemcpyHostToDevice, e->stream));
       memcpy(e->off_key, key, sizeof key);
    }
+   {
+      // smallest plane spacing 1/|row of h^-1|; cell diagonal bounds how far
+      // two sites in neighbouring listed cells can be from the cell centres
+      double sp = 1e300;
+      for (int r = 0; r < 3; r++) {
+         double n2 = e->hinv[3 * r] * e->hinv[3 * r] + e->hinv[3 * r + 1] * e->hinv[3 * r + 1] +
+                     e->hinv[3 * r + 2] * e->hinv[3 * r + 2];
+         sp = std::min(sp, 1.0 / sqrt(n2));
+      }
+      e->half_box = 0.5 * sp;
+      double cd = 0;
+      for (int r = 0; r < 3; r++) {
+         double c0 = h[3 * r] / nx, c1 = h[3 * r + 1] / ny, c2 = h[3 * r + 2] / nz;
+         cd += (fabs(c0) + fabs(c1) + fabs(c2)) * (fabs(c0) + fabs(c1) + fabs(c2));
+      }
+      cd = sqrt(cd);
+      // site binning: both sites of a pair lie within a cell diagonal of
+      // their cell centres; molecular binning puts a molecule in one entry
+      e->intra_exact = e->max_intra < e->half_box && (e->molpbc || e->max_intra + cd < e->cutoff);
+   }
    if (e->alpha > 1e-7) {
       double kkey[10] = {h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], h[8], e->k_cutoff};
       if (memcmp(kkey, e->kset_key, sizeof kkey) != 0 || e->nk == 0) {
@@ -314,6 +334,18 @@ void compute_statics(mgpu_engine *e, const double *chg) {
       isite += e->spec_nmols[s] * ns;
    }
    e->eintra = eintra;
+   double dmax = 0;
+   for (int sp = 0; sp < e->nspecies; sp++) {
+      if (e->spec_fw[sp]) continue;
+      const double *pf = e->spec_pf.data() + 3 * e->spec_off[sp];
+      for (int a = 0; a < e->spec_nsites[sp]; a++)
+         for (int b = a + 1; b < e->spec_nsites[sp]; b++) {
+            double d2 = 0;
+            for (int c = 0; c < 3; c++) d2 += (pf[3 * a + c] - pf[3 * b + c]) * (pf[3 * a + c] - pf[3 * b + c]);
+            dmax = std::max(dmax, sqrt(d2));
+         }
+   }
+   e->max_intra = dmax;
    // self and sheet energy, src/ewald.c:367-425
    double self = 0.0, sheet = 0.0, sq = 0.0, sqsq = 0.0;
    int ssite = 0, s = 0;
@@ -420,7 +452,8 @@ void real_pass(mgpu_engine *e, double *pe, double vir6[6], bool pre_virial, mgpu
    int tc[33];
    check_flags(e, tc);
    if (e->max_id > MGPU_MAX_ID) throw StatusError{MGPU_E_CAPACITY, "max_id > 16"};
-   *pe += v7[0] - e->eintra;
+   if (e->precision == MGPU_MIXED && e->intra_exact) *pe += v7[0];
+   else *pe += v7[0] - e->eintra;
    for (int q = 0; q < 6; q++) vir6[q] = v7[1 + q];
    (void)pre_virial;
    mgpu_fc_info fi{};
@@ -475,6 +508,38 @@ __global__ void k_pre_virial(int ns, int nsp, const double *__restrict__ s, cons
           ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
 }
 
+mgpu::ProfScope::ProfScope(mgpu_engine *e_, const char *n) : e(e_), name(n) {
+   if (!e->prof_on) return;
+   if (e->ev_used * 2 + 2 > (int)e->ev_pool.size()) {
+      for (int k = 0; k < 64; k++) {
+         hipEvent_t ev;
+         HIPCHK(hipEventCreate(&ev));
+         e->ev_pool.push_back(ev);
+      }
+   }
+   slot = e->ev_used++;
+   e->ev_names.push_back(name);
+   HIPCHK(hipEventRecord(e->ev_pool[2 * slot], e->stream));
+}
+
+mgpu::ProfScope::~ProfScope() {
+   if (slot >= 0) (void)hipEventRecord(e->ev_pool[2 * slot + 1], e->stream);
+}
+
+void mgpu::prof_resolve(mgpu_engine *e) {
+   if (!e->ev_used) return;
+   HIPCHK(hipStreamSynchronize(e->stream));
+   for (int s = 0; s < e->ev_used; s++) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e->ev_pool[2 * s], e->ev_pool[2 * s + 1]));
+      auto &p = e->prof[e->ev_names[s]];
+      p.total_ms += ms;
+      p.calls++;
+   }
+   e->ev_used = 0;
+   e->ev_names.clear();
+}
+
 // =============================== C ABI ===================================
 extern "C" {
 
@@ -513,6 +578,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
    fr(e->d_groups); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
    if (e->stream) (void)hipStreamDestroy(e->stream);
    delete e;
 }
@@ -788,11 +854,21 @@ int mgpu_profile_enable(mgpu_engine *e, int on) {
 }
 int mgpu_profile_reset(mgpu_engine *e) {
    if (!e) return MGPU_E_ARG;
+   int rc = guarded(e, [&] {
+      prof_resolve(e);
+      return 0;
+   });
    e->prof.clear();
-   return 0;
+   return rc;
 }
-int mgpu_profile_get(const mgpu_engine *e, const char *name, double *avg_ms, long *calls) {
-   if (!e || !name) return MGPU_E_ARG;
+int mgpu_profile_get(const mgpu_engine *ec, const char *name, double *avg_ms, long *calls) {
+   if (!ec || !name) return MGPU_E_ARG;
+   mgpu_engine *e = const_cast<mgpu_engine *>(ec);
+   int rc = guarded(e, [&] {
+      prof_resolve(e);
+      return 0;
+   });
+   if (rc) return rc;
    auto it = e->prof.find(name);
    if (it == e->prof.end()) {
       if (avg_ms) *avg_ms = 0;

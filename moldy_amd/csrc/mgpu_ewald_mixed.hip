@@ -1,7 +1,635 @@
-// Mixed-precision reciprocal-space sum (split-FP16 MFMA): placeholder until
-// the MFMA path is built; it refuses to run rather than falling back.
+// Mixed-precision reciprocal-space sum on the matrix cores.
+//
+// The k set of the reference (src/ewald.c:441-461) is organised as rows
+// r = (h,k) -- reciprocal vectors h a* + k b* -- each with a run of l.  With
+// A[r,i] = exp(i(h a*.r_i + k b*.r_i)) and Z[i,l] = q_i exp(i l c*.r_i), the
+// structure factor is a complex matrix product
+//     S[r,l] = sum_i A[r,i] Z[i,l]                     (ewald.c:525-531)
+// and, with W = coeff S (ewald.c:511, 543), the force back-projection
+// (ewald.c:557-566) is
+//     F_i = sum_r Im( A[r,i] (k_xy(r) P[i,r] + c*_z Q[i,r]) ),
+//     P[i,r] = sum_l Z[i,l] conj W[r,l],   Q[i,r] = sum_l l Z[i,l] conj W[r,l],
+// (plus kzt(r) P in the z component).  Both products run as
+// v_mfma_f32_16x16x32_f16 with every operand split into FP16 hi + lo parts
+// (a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi) and FP32 accumulation.  The l
+// range is padded to 16 NB columns; entries outside the reference's sphere
+// carry zero weight, so the result is the reference's sum over its k set.
+// The reduction over site slices, energy, stress and weights are FP64.
+//
+// Framework sites: site slices never mix framework and non-framework sites,
+// so S splits into its two parts as in ewald.c:528-540, and framework sites
+// are back-projected with the non-framework weights.
 #include "mgpu_internal.h"
 
-void mgpu::run_ewald_mixed(mgpu_engine *, double *) {
-   throw mgpu::StatusError{MGPU_E_STATE, "mixed-precision Ewald path not built"};
+#include <math.h>
+#include <algorithm>
+
+using namespace mgpu;
+
+namespace {
+
+typedef _Float16 half_t;
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr double PI_D = 3.14159265358979323846;
+constexpr int KS = 32;        // sites per MFMA k step
+constexpr int FROWS = 64;     // complex rows per forward workgroup
+constexpr int BSITES = 256;   // sites per backward workgroup
+
+__device__ __forceinline__ void split2(double v, half_t &hi, half_t &lo) {
+   float f = (float)v;
+   hi = (half_t)f;
+   lo = (half_t)(float)(v - (double)(float)hi);
+}
+__device__ __forceinline__ void split2f(float v, half_t &hi, half_t &lo) {
+   hi = (half_t)v;
+   lo = (half_t)(v - (float)hi);
+}
+
+// ------------------------------------------------------------- tables
+struct TabArgs {
+   double ks[9];
+   int hmax, kmax, lmax, ns, nsp;
+   double sZ;
+};
+
+// exp(i n k*.r) by the reference's recurrence (ewald.c:148-193), FP64, then
+// FP32 for the x and y axes and split FP16 for q exp(i l c*.r).
+__global__ void k_mix_tables(TabArgs a, const double *__restrict__ site, const double *__restrict__ chg,
+                             float2 *__restrict__ X, float2 *__restrict__ Y, half_t *__restrict__ Zhr,
+                             half_t *__restrict__ Zlr, half_t *__restrict__ Zhi, half_t *__restrict__ Zli) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= a.nsp) return;
+   if (i >= a.ns) {
+      for (int n = 0; n <= a.hmax; n++) X[(size_t)n * a.nsp + i] = make_float2(0, 0);
+      for (int n = 0; n <= a.kmax; n++) Y[(size_t)n * a.nsp + i] = make_float2(0, 0);
+      for (int n = 0; n <= a.lmax; n++) {
+         size_t o = (size_t)n * a.nsp + i;
+         Zhr[o] = Zlr[o] = Zhi[o] = Zli[o] = (half_t)0;
+      }
+      return;
+   }
+   double x = site[i], y = site[a.nsp + i], z = site[2 * a.nsp + i], q = chg[i] * a.sZ;
+   for (int ax = 0; ax < 3; ax++) {
+      int nmax = ax == 0 ? a.hmax : (ax == 1 ? a.kmax : a.lmax);
+      double kr = a.ks[3 * ax] * x + a.ks[3 * ax + 1] * y + a.ks[3 * ax + 2] * z;
+      double s1, c1;
+      sincos(kr, &s1, &c1);
+      double cn = 1.0, sn = 0.0;
+      for (int n = 0; n <= nmax; n++) {
+         size_t o = (size_t)n * a.nsp + i;
+         if (ax == 0) X[o] = make_float2((float)cn, (float)sn);
+         else if (ax == 1) Y[o] = make_float2((float)cn, (float)sn);
+         else {
+            split2(q * cn, Zhr[o], Zlr[o]);
+            split2(q * sn, Zhi[o], Zli[o]);
+         }
+         double cc = cn * c1 - sn * s1;
+         sn = sn * c1 + cn * s1;
+         cn = cc;
+      }
+   }
+}
+
+// ---------------------------------------------------- forward: S = A Z
+struct SkArgs {
+   int ntile, lmax, nsp;
+   const int2 *rowhk;     // [ntile*64] (h, k); h < 0 marks padding
+   const float2 *X, *Y;
+   const half_t *Zhr, *Zlr, *Zhi, *Zli;
+   const int4 *slices;    // (i0 aligned to 32, i1, valid lo, valid hi)
+   float *part;           // [nslices][ntile][4][64][16 NB]
+};
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
+   constexpr int NCOL = 16 * NB;
+   __shared__ __attribute__((aligned(16))) half_t sA[4][FROWS][KS + 8];
+   __shared__ int2 shk[FROWS];
+   const int tile = blockIdx.x, slice = blockIdx.y;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int wm = wave >> 1, wn = wave & 1, g = lane >> 4, lc = lane & 15;
+   if (tid < FROWS) shk[tid] = a.rowhk[tile * FROWS + tid];
+   const int4 sl = a.slices[slice];
+   floatx4 acc[4][NB];
+#pragma unroll
+   for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int n = 0; n < NB; n++) acc[m][n] = floatx4{0, 0, 0, 0};
+   const half_t *Zh = wn == 0 ? a.Zhr : a.Zhi;
+   const half_t *Zl = wn == 0 ? a.Zlr : a.Zli;
+   const int si = tid & 31, rg = tid >> 5;
+   for (int i0 = sl.x; i0 < sl.y; i0 += KS) {
+      __syncthreads();
+      {
+         const int i = i0 + si;
+         const bool valid = i >= sl.z && i < sl.w;
+#pragma unroll
+         for (int j = 0; j < 8; j++) {
+            const int rr = rg + 8 * j;
+            const int2 hk = shk[rr];
+            float ar = 0.f, ai = 0.f;
+            if (hk.x >= 0 && valid) {
+               float2 x = a.X[(size_t)hk.x * a.nsp + i];
+               float2 y = a.Y[(size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp + i];
+               if (hk.y < 0) y.y = -y.y;
+               ar = x.x * y.x - x.y * y.y;
+               ai = x.y * y.x + x.x * y.y;
+            }
+            split2f(ar, sA[0][rr][si], sA[1][rr][si]);
+            split2f(ai, sA[2][rr][si], sA[3][rr][si]);
+         }
+      }
+      __syncthreads();
+      half8 bh[NB], bl[NB];
+#pragma unroll
+      for (int n = 0; n < NB; n++) {
+         const int l = n * 16 + lc - a.lmax;
+         const int al = l < 0 ? -l : l;
+         if (al <= a.lmax) {
+            const size_t o = (size_t)al * a.nsp + i0 + 8 * g;
+            bh[n] = *(const half8 *)(Zh + o);
+            bl[n] = *(const half8 *)(Zl + o);
+            if (wn == 1 && l < 0) {
+               bh[n] = -bh[n];
+               bl[n] = -bl[n];
+            }
+         } else {
+            bh[n] = half8{0, 0, 0, 0, 0, 0, 0, 0};
+            bl[n] = bh[n];
+         }
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+         const half8 ah = *(const half8 *)&sA[2 * wm][m * 16 + lc][8 * g];
+         const half8 al8 = *(const half8 *)&sA[2 * wm + 1][m * 16 + lc][8 * g];
+#pragma unroll
+         for (int n = 0; n < NB; n++) {
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8, bh[n], acc[m][n], 0, 0, 0);
+         }
+      }
+   }
+   float *out = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wm * 2 + wn) * FROWS * NCOL;
+#pragma unroll
+   for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int n = 0; n < NB; n++)
+#pragma unroll
+         for (int j = 0; j < 4; j++) out[(m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = acc[m][n][j];
+}
+
+// --------------------- slice reduction, energy, stress and weights (FP64)
+struct RedArgs {
+   int nrows, ntile, nslices, lmax, ncol, nk;
+   double vol, r4a, c2, inv_sZ;
+   const KRow *rows;
+   const int *slice_fw;
+   const float *part;
+   double *Wd, *Wnd;      // dense [nrows][ncol][2]
+   double *w;             // per k: Wc, Ws, Wcn, Wsn
+   double *bpart;         // per block: 7 sums
+   unsigned *maxbits;     // [0] max|W|, [1] max|l W| as float bits
+};
+
+__global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
+   __shared__ double ws[4][8];
+   const int idx = blockIdx.x * 256 + threadIdx.x;
+   double v[7] = {0, 0, 0, 0, 0, 0, 0};
+   float mw = 0.f, mq = 0.f;
+   if (idx < a.nrows * a.ncol) {
+      const int r = idx / a.ncol, c = idx - r * a.ncol, l = c - a.lmax;
+      const KRow R = a.rows[r];
+      double wr = 0, wi = 0, wnr = 0, wni = 0;
+      if (l >= R.l0 && l < R.l0 + R.nl) {
+         const int tile = r / FROWS, rr = r - tile * FROWS;
+         double scn = 0, ssn = 0, scf = 0, ssf = 0;
+         const size_t pl = (size_t)FROWS * a.ncol;
+         for (int s = 0; s < a.nslices; s++) {
+            const float *p = a.part + ((size_t)(s * a.ntile + tile) * 4) * pl + rr * a.ncol + c;
+            const double sr = (double)p[0] - (double)p[3 * pl];
+            const double si = (double)p[pl] + (double)p[2 * pl];
+            if (a.slice_fw[s]) { scf += sr; ssf += si; }
+            else { scn += sr; ssn += si; }
+         }
+         scn *= a.inv_sZ; ssn *= a.inv_sZ; scf *= a.inv_sZ; ssf *= a.inv_sZ;
+         const double kx = R.kx, ky = R.ky, kz = R.kzt + l * a.c2;
+         const double ksq = kx * kx + ky * ky + kz * kz;
+         const double coeff = 2.0 / ((0.25 / PI_D) * a.vol) * exp(ksq * a.r4a) / ksq;
+         const double coeff2 = 2.0 * (1.0 - ksq * a.r4a) / ksq;
+         const double pe_k = 0.5 * coeff * (scn * (scn + scf + scf) + ssn * (ssn + ssf + ssf));
+         v[0] = pe_k;
+         v[1] = pe_k - pe_k * coeff2 * kx * kx;
+         v[2] = -pe_k * coeff2 * kx * ky;
+         v[3] = -pe_k * coeff2 * kx * kz;
+         v[4] = pe_k - pe_k * coeff2 * ky * ky;
+         v[5] = -pe_k * coeff2 * ky * kz;
+         v[6] = pe_k - pe_k * coeff2 * kz * kz;
+         wr = coeff * (scn + scf); wi = coeff * (ssn + ssf);
+         wnr = coeff * scn; wni = coeff * ssn;
+         const int k = R.kstart + (l - R.l0);
+         double *wk = a.w + 4 * (size_t)k;
+         wk[0] = wr; wk[1] = wi; wk[2] = wnr; wk[3] = wni;
+         mw = (float)fmax(fmax(fabs(wr), fabs(wi)), fmax(fabs(wnr), fabs(wni)));
+         mq = mw * (float)(l < 0 ? -l : l);
+      }
+      double *d = a.Wd + 2 * (size_t)idx;
+      d[0] = wr; d[1] = wi;
+      double *dn = a.Wnd + 2 * (size_t)idx;
+      dn[0] = wnr; dn[1] = wni;
+   }
+   // block maxima (positive floats compare as their bit patterns)
+   for (int o = 32; o > 0; o >>= 1) {
+      mw = fmaxf(mw, __shfl_xor(mw, o, 64));
+      mq = fmaxf(mq, __shfl_xor(mq, o, 64));
+   }
+   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+   if (lane == 0) {
+      if (mw > 0) atomicMax(&a.maxbits[0], __float_as_uint(mw));
+      if (mq > 0) atomicMax(&a.maxbits[1], __float_as_uint(mq));
+   }
+#pragma unroll
+   for (int q = 0; q < 7; q++) {
+      double t = wave_sum(v[q]);
+      if (lane == 0) ws[wv][q] = t;
+   }
+   __syncthreads();
+   if (threadIdx.x < 7)
+      a.bpart[8 * (size_t)blockIdx.x + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+__device__ __forceinline__ float pow2_scale(unsigned bits, float target) {
+   float m = __uint_as_float(bits);
+   if (!(m > 0.f)) return 1.f;
+   return exp2f(floorf(log2f(target / m)));
+}
+
+// ------------------- weight operand tables for the back-projection
+struct WtArgs {
+   int nrows, nrows16, ncol, lmax;
+   const double *Wd;
+   half_t *Wt;            // [4 kinds][2 hi/lo][nrows16][2 ncol]
+   const unsigned *maxbits;
+};
+
+__global__ void k_mix_wtab(WtArgs a) {
+   const int K2 = 2 * a.ncol;
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= a.nrows16 * K2) return;
+   const int r = idx / K2, kk = idx - r * K2;
+   const int part = kk >= a.ncol, c = kk - part * a.ncol, l = c - a.lmax;
+   double wr = 0, wi = 0;
+   if (r < a.nrows) {
+      wr = a.Wd[2 * ((size_t)r * a.ncol + c)];
+      wi = a.Wd[2 * ((size_t)r * a.ncol + c) + 1];
+   }
+   const double sW = pow2_scale(a.maxbits[0], 8192.f), sQ = pow2_scale(a.maxbits[1], 8192.f);
+   // P = sum_l Z conj(W): Pr = Zr Wr + Zi Wi, Pi = Zi Wr - Zr Wi; Q likewise with l W
+   double v[4];
+   if (!part) { v[0] = wr; v[1] = -wi; v[2] = l * wr; v[3] = -l * wi; }
+   else { v[0] = wi; v[1] = wr; v[2] = l * wi; v[3] = l * wr; }
+   for (int kind = 0; kind < 4; kind++) {
+      const double s = kind < 2 ? sW : sQ;
+      half_t hi, lo;
+      split2(v[kind] * s, hi, lo);
+      a.Wt[((size_t)(kind * 2 + 0) * a.nrows16 + r) * K2 + kk] = hi;
+      a.Wt[((size_t)(kind * 2 + 1) * a.nrows16 + r) * K2 + kk] = lo;
+   }
+}
+
+// -------------------------------------- back-projection: forces on sites
+struct FArgs {
+   int s0, s1, nrows, nrows16, lmax, kmax, nsp;
+   float c2, sZ;
+   const int2 *rowhk;       // [nrows16] (h, k)
+   const float4 *rowk;      // [nrows16] (kx, ky, kzt, 0)
+   const float2 *X, *Y;
+   const half_t *Zhr, *Zlr, *Zhi, *Zli;
+   const half_t *Wt;
+   const unsigned *maxbits;
+   double *force;           // [3][nsp]
+};
+
+template <int NB>
+__global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
+   constexpr int NCOL = 16 * NB, K2 = 2 * NCOL;
+   __shared__ __attribute__((aligned(16))) half_t sW[4][2][16][K2 + 8];
+   extern __shared__ float2 sY[];   // [(kmax+1)][BSITES]
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
+   const int base = a.s0 + blockIdx.x * BSITES;
+   for (int q = tid; q < (a.kmax + 1) * BSITES; q += 512) {
+      const int k = q / BSITES, t = q - k * BSITES, i = base + t;
+      sY[q] = i < a.s1 ? a.Y[(size_t)k * a.nsp + i] : make_float2(0, 0);
+   }
+   // Z operand [site][Zr | Zi over the padded l range], held for the whole kernel
+   half8 zh[2][NB], zl[2][NB];
+#pragma unroll
+   for (int m = 0; m < 2; m++) {
+      const int i = base + 32 * wave + 16 * m + lc;
+#pragma unroll
+      for (int ks = 0; ks < NB; ks++) {
+#pragma unroll
+         for (int e = 0; e < 8; e++) {
+            const int kk = ks * 32 + 8 * g + e;
+            const bool im = kk >= NCOL;
+            const int l = (im ? kk - NCOL : kk) - a.lmax, al = l < 0 ? -l : l;
+            half_t h = (half_t)0, lo = (half_t)0;
+            if (al <= a.lmax && i < a.s1) {
+               const size_t o = (size_t)al * a.nsp + i;
+               h = im ? a.Zhi[o] : a.Zhr[o];
+               lo = im ? a.Zli[o] : a.Zlr[o];
+               if (im && l < 0) { h = -h; lo = -lo; }
+            }
+            zh[m][ks][e] = h;
+            zl[m][ks][e] = lo;
+         }
+      }
+   }
+   const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
+   const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
+   float f[2][4][3];
+#pragma unroll
+   for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
+   int cur_h = -1;
+   float2 xv[2][4];
+   __syncthreads();
+   const int ntile = a.nrows16 / 16;
+   for (int rt = 0; rt < ntile; rt++) {
+      // stage the weight tile: 4 kinds x hi/lo x 16 rows x K2
+      constexpr int CH = 4 * 2 * 16 * (K2 / 8);
+      for (int q = tid; q < CH; q += 512) {
+         const int c8 = q % (K2 / 8), row = (q / (K2 / 8)) % 16, kh = q / (16 * (K2 / 8));
+         *(half8 *)&sW[kh >> 1][kh & 1][row][8 * c8] =
+             *(const half8 *)(a.Wt + ((size_t)kh * a.nrows16 + rt * 16 + row) * K2 + 8 * c8);
+      }
+      __syncthreads();
+      floatx4 acc[2][4];
+#pragma unroll
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+         for (int kd = 0; kd < 4; kd++) acc[m][kd] = floatx4{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < NB; ks++) {
+#pragma unroll
+         for (int kd = 0; kd < 4; kd++) {
+            const half8 bh = *(const half8 *)&sW[kd][0][lc][ks * 32 + 8 * g];
+            const half8 bl = *(const half8 *)&sW[kd][1][lc][ks * 32 + 8 * g];
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
+               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
+               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
+            }
+         }
+      }
+      const int r = rt * 16 + lc;
+      if (r < a.nrows) {
+         const int2 hk = a.rowhk[r];
+         const float4 kv = a.rowk[r];
+         if (hk.x != cur_h) {
+            cur_h = hk.x;
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+               for (int j = 0; j < 4; j++) {
+                  const int i = base + 32 * wave + 16 * m + 4 * g + j;
+                  xv[m][j] = i < a.s1 ? a.X[(size_t)hk.x * a.nsp + i] : make_float2(0, 0);
+               }
+         }
+         const int ak = hk.y < 0 ? -hk.y : hk.y;
+         const float ysg = hk.y < 0 ? -1.f : 1.f;
+#pragma unroll
+         for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+               const int t = 32 * wave + 16 * m + 4 * g + j;
+               float2 y = sY[ak * BSITES + t];
+               y.y *= ysg;
+               const float2 x = xv[m][j];
+               const float Ar = x.x * y.x - x.y * y.y, Ai = x.y * y.x + x.x * y.y;
+               const float Pr = acc[m][0][j] * inv_pw, Pi = acc[m][1][j] * inv_pw;
+               const float Qr = acc[m][2][j] * inv_qw, Qi = acc[m][3][j] * inv_qw;
+               const float t1 = Ar * Pi + Ai * Pr;
+               const float t2 = Ar * Qi + Ai * Qr;
+               f[m][j][0] += kv.x * t1;
+               f[m][j][1] += kv.y * t1;
+               f[m][j][2] += kv.z * t1 + a.c2 * t2;
+            }
+      }
+      __syncthreads();
+   }
+   // lanes with the same g hold the same sites: sum over the 16 row lanes
+#pragma unroll
+   for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+         for (int c = 0; c < 3; c++) {
+            float v = f[m][j][c];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            f[m][j][c] = v;
+         }
+   if (lc == 0) {
+#pragma unroll
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+         for (int j = 0; j < 4; j++) {
+            const int i = base + 32 * wave + 16 * m + 4 * g + j;
+            if (i < a.s1) {
+               a.force[i] += (double)f[m][j][0];
+               a.force[a.nsp + i] += (double)f[m][j][1];
+               a.force[2 * a.nsp + i] += (double)f[m][j][2];
+            }
+         }
+   }
+}
+
+__global__ void k_mix_sum(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
+   __shared__ double ws[16];
+   int col = blockIdx.x;
+   double s = 0;
+   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
+   s = wave_sum(s);
+   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
+      out[col] = t;
+   }
+}
+
+template <int NB> void launch_sk(dim3 g, hipStream_t s, const SkArgs &a) {
+   k_mix_sk<NB><<<g, 256, 0, s>>>(a);
+}
+template <int NB> void launch_force(dim3 g, size_t shm, hipStream_t s, const FArgs &a) {
+   static bool attr = false;
+   if (!attr) {
+      HIPCHK(hipFuncSetAttribute((const void *)k_mix_force<NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 96 * 1024));
+      attr = true;
+   }
+   k_mix_force<NB><<<g, 512, shm, s>>>(a);
+}
+
+#define NB_SWITCH(nb, CALL)                                                     \
+   switch (nb) {                                                                \
+   case 1: CALL(1); break;                                                      \
+   case 2: CALL(2); break;                                                      \
+   case 3: CALL(3); break;                                                      \
+   case 4: CALL(4); break;                                                      \
+   case 5: CALL(5); break;                                                      \
+   case 6: CALL(6); break;                                                      \
+   default: throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path"}; \
+   }
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
+   const int ns = e->nsites, nsp = e->nsp, lmax = e->lmax;
+   const int NB = (2 * lmax + 1 + 15) / 16;
+   if (NB > 6) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (lmax > 47)"};
+   if ((e->kmax + 1) * BSITES * sizeof(float2) > 96 * 1024)
+      throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (kmax > 47)"};
+   const int NCOL = 16 * NB, K2 = 2 * NCOL;
+   const int nrows = e->nrows;
+   const int ntile = (nrows + FROWS - 1) / FROWS;
+   const int nrows64 = ntile * FROWS, nrows16 = nrows64;
+   // site slices: 32-aligned starts, never mixing framework and other sites
+   std::vector<int4> slices;
+   std::vector<int> slice_fw;
+   {
+      int want = std::max(1, std::min(64, (1024 + ntile - 1) / ntile));
+      auto add_range = [&](int lo, int hi, int fw) {
+         if (hi <= lo) return;
+         int n = hi - lo, per = std::max(KS, ((n + want - 1) / want + KS - 1) / KS * KS);
+         for (int s = lo; s < hi; s += per) {
+            int a0 = s & ~(KS - 1), a1 = std::min(hi, s + per);
+            slices.push_back(make_int4(a0, a1, s, a1));
+            slice_fw.push_back(fw);
+         }
+      };
+      add_range(0, e->nsitesxf, 0);
+      add_range(e->nsitesxf, ns, 1);
+   }
+   const int nslices = (int)slices.size();
+   // scratch layout
+   size_t off = 0;
+   auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return o; };
+   const size_t oX = take(sizeof(float2) * (e->hmax + 1) * (size_t)nsp);
+   const size_t oY = take(sizeof(float2) * (e->kmax + 1) * (size_t)nsp);
+   const size_t oZ = take(sizeof(half_t) * 4 * (lmax + 1) * (size_t)nsp);
+   const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 4 * FROWS * NCOL);
+   const size_t oWd = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
+   const size_t oWn = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
+   const size_t oWt = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
+   const size_t oWtn = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
+   const size_t oHK = take(sizeof(int2) * nrows64);
+   const size_t oRK = take(sizeof(float4) * nrows64);
+   const size_t oSl = take(sizeof(int4) * nslices);
+   const size_t oSf = take(sizeof(int) * nslices);
+   const size_t oMx = take(sizeof(unsigned) * 4);
+   const int nred = (nrows * NCOL + 255) / 256;
+   const size_t oBp = take(sizeof(double) * 8 * (size_t)nred);
+   e->d_mix.reserve(off);
+   uint8_t *B = e->d_mix.p;
+   float2 *X = (float2 *)(B + oX), *Y = (float2 *)(B + oY);
+   half_t *Zhr = (half_t *)(B + oZ), *Zlr = Zhr + (size_t)(lmax + 1) * nsp,
+          *Zhi = Zlr + (size_t)(lmax + 1) * nsp, *Zli = Zhi + (size_t)(lmax + 1) * nsp;
+   float *part = (float *)(B + oP);
+   double *Wd = (double *)(B + oWd), *Wnd = (double *)(B + oWn);
+   half_t *Wt = (half_t *)(B + oWt), *Wtn = (half_t *)(B + oWtn);
+   int2 *rowhk = (int2 *)(B + oHK);
+   float4 *rowk = (float4 *)(B + oRK);
+   int4 *dsl = (int4 *)(B + oSl);
+   int *dsf = (int *)(B + oSf);
+   unsigned *maxbits = (unsigned *)(B + oMx);
+   double *bpart = (double *)(B + oBp);
+   // row metadata (host, from the k set)
+   {
+      std::vector<int2> hk(nrows64, make_int2(-1, 0));
+      std::vector<float4> rk(nrows64, make_float4(0, 0, 0, 0));
+      for (int r = 0; r < nrows; r++) {
+         hk[r] = make_int2(e->rows[r].h, e->rows[r].k);
+         rk[r] = make_float4((float)e->rows[r].kx, (float)e->rows[r].ky, (float)e->rows[r].kzt, 0.f);
+      }
+      HIPCHK(hipMemcpyAsync(rowhk, hk.data(), sizeof(int2) * nrows64, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(rowk, rk.data(), sizeof(float4) * nrows64, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(dsl, slices.data(), sizeof(int4) * nslices, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(dsf, slice_fw.data(), sizeof(int) * nslices, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemsetAsync(maxbits, 0, 4 * sizeof(unsigned), e->stream));
+   }
+   double qmax = 0;
+   for (double q : e->site_charge) qmax = std::max(qmax, fabs(q));
+   const double sZ = qmax > 0 ? exp2(-ceil(log2(qmax))) : 1.0;
+   {
+      ProfScope ps(e, "ewald_mfma_prep");
+      TabArgs t;
+      for (int i = 0; i < 9; i++) t.ks[i] = 2 * PI_D * e->hinv[i];
+      t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = lmax; t.ns = ns; t.nsp = nsp; t.sZ = sZ;
+      k_mix_tables<<<(nsp + 255) / 256, 256, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Zhr,
+                                                             Zlr, Zhi, Zli);
+   }
+   {
+      ProfScope ps(e, "ewald_mfma_sk");
+      SkArgs s{ntile, lmax, nsp, rowhk, X, Y, Zhr, Zlr, Zhi, Zli, dsl, part};
+      dim3 g(ntile, nslices);
+#define CALL_SK(nb) launch_sk<nb>(g, e->stream, s)
+      NB_SWITCH(NB, CALL_SK)
+#undef CALL_SK
+   }
+   e->d_wk.reserve((size_t)e->nk * 4 + 4);
+   {
+      ProfScope ps(e, "ewald_mfma_reduce");
+      RedArgs r;
+      r.nrows = nrows; r.ntile = ntile; r.nslices = nslices; r.lmax = lmax; r.ncol = NCOL; r.nk = e->nk;
+      r.vol = host_det(e->h);
+      r.r4a = -1.0 / (4.0 * e->alpha * e->alpha);
+      r.c2 = 2 * PI_D * e->hinv[8];
+      r.inv_sZ = 1.0 / sZ;
+      r.rows = e->d_rows.p; r.slice_fw = dsf; r.part = part; r.Wd = Wd; r.Wnd = Wnd; r.w = e->d_wk.p;
+      r.bpart = bpart; r.maxbits = maxbits;
+      k_mix_reduce<<<nred, 256, 0, e->stream>>>(r);
+      WtArgs w{nrows, nrows16, NCOL, lmax, Wd, Wt, maxbits};
+      int nw = nrows16 * K2;
+      k_mix_wtab<<<(nw + 255) / 256, 256, 0, e->stream>>>(w);
+      if (e->nsitesxf < ns) {
+         WtArgs wn{nrows, nrows16, NCOL, lmax, Wnd, Wtn, maxbits};
+         k_mix_wtab<<<(nw + 255) / 256, 256, 0, e->stream>>>(wn);
+      }
+   }
+   {
+      ProfScope ps(e, "ewald_mfma_force");
+      FArgs f;
+      f.nrows = nrows; f.nrows16 = nrows16; f.lmax = lmax; f.kmax = e->kmax; f.nsp = nsp;
+      f.c2 = (float)(2 * PI_D * e->hinv[8]);
+      f.sZ = (float)sZ;
+      f.rowhk = rowhk; f.rowk = rowk; f.X = X; f.Y = Y;
+      f.Zhr = Zhr; f.Zlr = Zlr; f.Zhi = Zhi; f.Zli = Zli; f.maxbits = maxbits;
+      f.force = e->d_force_site.p;
+      const size_t shm = sizeof(float2) * (e->kmax + 1) * BSITES;
+      auto run = [&](int s0, int s1, const half_t *wt) {
+         if (s1 <= s0) return;
+         f.s0 = s0; f.s1 = s1; f.Wt = wt;
+         dim3 g((s1 - s0 + BSITES - 1) / BSITES);
+#define CALL_F(nb) launch_force<nb>(g, shm, e->stream, f)
+         NB_SWITCH(NB, CALL_F)
+#undef CALL_F
+      };
+      run(0, e->nsitesxf, Wt);
+      run(e->nsitesxf, ns, Wtn);
+   }
+   e->d_out.reserve(64);
+   k_mix_sum<<<7, 256, 0, e->stream>>>(bpart, nred, 8, e->d_out.p + 24);
+   HIPCHK(hipMemcpyAsync(out7, e->d_out.p + 24, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
 }

@@ -85,6 +85,9 @@ struct mgpu_engine {
    std::vector<int> entry_isite, entry_num, entry_ftype, entry_mol;  // binning entries
    int nentries = 0, n_frame_types = 1;
    double eintra = 0, self_energy = 0, sheet_energy = 0, dist = 0, distp = 0;
+   double max_intra = 0;       // largest intramolecular site distance
+   double half_box = 0;        // half the smallest plane spacing of h
+   bool intra_exact = false;   // every intramolecular pair is listed exactly once
    bool statics_done = false;
    int nsp = 0;        // padded site stride
 
@@ -148,6 +151,9 @@ struct mgpu_engine {
    mgpu_ew_info last_ew{};
    bool prof_on = false;
    std::map<std::string, mgpu::KProf> prof;
+   std::vector<hipEvent_t> ev_pool;             // pairs: 2*slot, 2*slot+1
+   std::vector<std::string> ev_names;           // pending slot names
+   int ev_used = 0;
 };
 
 namespace mgpu {
@@ -168,31 +174,16 @@ void sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *ho
 double host_det(const double a[9]);
 void host_invert(const double a[9], double b[9]);
 
+// Event timing that does not stall the stream: events come from a pool and
+// are resolved when the profile is read (mgpu_profile_get).
 struct ProfScope {
    mgpu_engine *e;
    const char *name;
-   hipEvent_t a{}, b{};
-   ProfScope(mgpu_engine *e_, const char *n) : e(e_), name(n) {
-      if (e->prof_on) {
-         HIPCHK(hipEventCreate(&a));
-         HIPCHK(hipEventCreate(&b));
-         HIPCHK(hipEventRecord(a, e->stream));
-      }
-   }
-   ~ProfScope() {
-      if (e->prof_on) {
-         float ms = 0;
-         (void)hipEventRecord(b, e->stream);
-         (void)hipEventSynchronize(b);
-         (void)hipEventElapsedTime(&ms, a, b);
-         auto &p = e->prof[name];
-         p.total_ms += ms;
-         p.calls++;
-         (void)hipEventDestroy(a);
-         (void)hipEventDestroy(b);
-      }
-   }
+   int slot = -1;
+   ProfScope(mgpu_engine *e_, const char *n);
+   ~ProfScope();
 };
+void prof_resolve(mgpu_engine *e);
 }  // namespace mgpu
 
 // ---------------------------------------------------------- device helpers

@@ -105,7 +105,8 @@ __device__ __forceinline__ void pair_term(R rsq, R qq, R norm, R alpha, const R 
 
 struct RSArgs {
    int ncells, nx, ny, nz, nfull, max_id, nsp, strict_sub;
-   double alpha, norm, cutsq, cut100sq;
+   int skip_intra_u;   // FP32: drop minimum-image intramolecular pair energies
+   double alpha, norm, cutsq, cut100sq, half_box_sq;
    double reloc[81];
    const int *css;
    const int4 *off;
@@ -235,6 +236,11 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
                pair_term<R, PT, COUL>(rsq, qi[k] * qj, norm, alpha,
                                       &spot[(ti[k] * a.max_id + tj) * MGPU_NPOTP], u, f);
                if (skip) { u = 0; f = 0; }
+               // The minimum-image intramolecular pairs of rigid molecules sum
+               // to eintra, which the FP64 path subtracts again; the FP32 path
+               // leaves both out so their large cancelling energies never
+               // meet FP32 rounding.  Forces are kept.
+               if (sizeof(R) == 4 && a.skip_intra_u && mi[k] == mj.y && rsq < (R)a.half_box_sq) u = 0;
                fxa[k] -= f * rx;
                fya[k] -= f * ry;
                fza[k] -= f * rz;
@@ -317,6 +323,8 @@ void mgpu::run_realspace(mgpu_engine *e, double *out7) {
    a.norm = 2.0 * e->alpha / sqrt(3.14159265358979323846);
    a.cutsq = e->cutoff * e->cutoff;
    a.cut100sq = 10000.0 * a.cutsq;
+   a.skip_intra_u = e->precision == MGPU_MIXED && e->intra_exact;
+   a.half_box_sq = e->half_box * e->half_box;
    int k = 0;
    for (int ii = -1; ii <= 1; ii++)
       for (int jj = -1; jj <= 1; jj++)
