@@ -1,0 +1,281 @@
+/*
+ * force_shim.c -- Moldy's force_calc()/ewald()/poteval() on the MI355X engine.
+ *
+ * Converts Moldy's structures (system_mt, spec_mt[], pot_mt[], the contr_mt
+ * fields the path reads) into the flat mgpu_system of libmoldy_amd.so once,
+ * then forwards each call with Moldy's site/site_force blocks ([3][nsarray],
+ * src/accel.c:422-426).  Results accumulate exactly as the reference's do.
+ * Notes, warnings and fatal errors go through Moldy's note()/message(), with
+ * the texts the reference prints (src/messages.h), so the output file keeps
+ * its content.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "moldy_amd.h"
+#include "moldy_force.h"
+
+/* ---- Moldy's globals and reporting; weak so that Moldy's own win ---- */
+__attribute__((weak)) contr_mt control;
+__attribute__((weak)) int ithread = 0;
+__attribute__((weak)) int nthreads = 1;
+
+#define SEV_INFO 0
+#define SEV_WARNING 1
+#define SEV_ERROR 2
+#define SEV_FATAL 3
+
+__attribute__((weak)) void note(char *text, ...)
+{
+   va_list ap;
+   if (ithread > 0)
+      return;
+   va_start(ap, text);
+   printf(" *I* ");
+   vprintf(text, ap);
+   printf("\n");
+   va_end(ap);
+}
+
+__attribute__((weak)) void message(int *nerrs, ...)
+{
+   static const char *sev_txt[] = {" *I* ", " *W* ", " *E* ", " *F* "};
+   va_list ap;
+   char *buff, *format;
+   int sev;
+   va_start(ap, nerrs);
+   buff = va_arg(ap, char *);
+   sev = va_arg(ap, int);
+   format = va_arg(ap, char *);
+   if (ithread == 0 || sev == SEV_FATAL) {
+      printf("%s", sev_txt[sev < 0 ? 0 : (sev > 3 ? 3 : sev)]);
+      vprintf(format, ap);
+      printf("\n");
+      if (buff)
+         printf("     buffer contents=\"%s\"\n", buff);
+   }
+   va_end(ap);
+   if (sev >= SEV_ERROR && nerrs)
+      (*nerrs)++;
+   if (sev == SEV_FATAL) {
+      fflush(stdout);
+      exit(3);
+   }
+}
+
+/* ------------------------------------------------------------- engine */
+static mgpu_engine *eng = NULL;
+static int sys_nsites = -1, sys_nmols = -1, sys_have_pot = 0;
+static int onx = 0, ony = 0, onz = 0, onabor = 0;
+static int fc_init = 1, ew_init = 1;
+
+#define CONV_E (0.001 * 6.0221367e23 * 1.6605402e-27 * 1.0e4) /* kJ/mol, src/defs.h:232 */
+#define CONV_Q (4.07497263794495e-14 * 1.e-3 * 1.05482230112e-05 / 1.60217733e-19)
+
+static void fatal_mgpu(int rc)
+{
+   const char *msg = eng ? mgpu_last_error(eng) : "engine not created";
+   switch (rc) {
+   case MGPU_E_BIN_RANGE:
+   case MGPU_E_BIN_ROUND:
+      message(NULL, NULL, SEV_ERROR, "%s", (char *)msg);
+      return;
+   case MGPU_E_CUTOFF:
+      /* src/messages.h:94 */
+      message(NULL, NULL, SEV_FATAL,
+              "Cutoff radius > %d * cell dimension.\n   Increase the MD cell size or reduce the cutoff", 1);
+      return;
+   default:
+      message(NULL, NULL, SEV_FATAL, "MI355X force engine: %s (status %d)", (char *)msg, rc);
+   }
+}
+
+static void ensure_engine(system_mt *system, spec_mt *species, real *chg, pot_mt *potpar)
+{
+   int rc, i, s, nsp_sites = 0, max_id = system->max_id;
+   int *spn, *sps, *spf, *spr, *sid;
+   double *pf, *qid, *pp;
+   mgpu_system d;
+   const char *prec;
+   if (eng && system->nsites == sys_nsites && system->nmols == sys_nmols &&
+       (sys_have_pot || !potpar))
+      return;
+   if (!eng) {
+      const char *dev = getenv("MOLDY_AMD_DEVICE");
+      rc = mgpu_create(&eng, dev ? atoi(dev) : 0);
+      if (rc)
+         fatal_mgpu(rc);
+   }
+   for (s = 0; s < system->nspecies; s++)
+      nsp_sites += species[s].nsites;
+   spn = (int *)malloc(sizeof(int) * 4 * system->nspecies);
+   sps = spn + system->nspecies;
+   spf = sps + system->nspecies;
+   spr = spf + system->nspecies;
+   sid = (int *)malloc(sizeof(int) * (nsp_sites + 1));
+   pf = (double *)malloc(sizeof(double) * 3 * (nsp_sites + 1));
+   qid = (double *)calloc(max_id, sizeof(double));
+   pp = (double *)malloc(sizeof(double) * MOLDY_NPOTP * max_id * max_id);
+   {
+      int k = 0, isite = 0;
+      for (s = 0; s < system->nspecies; s++) {
+         spn[s] = species[s].nmols;
+         sps[s] = species[s].nsites;
+         spf[s] = species[s].framework;
+         spr[s] = species[s].rdof;
+         for (i = 0; i < species[s].nsites; i++, k++) {
+            sid[k] = species[s].site_id[i];
+            pf[3 * k] = species[s].p_f_sites[i][0];
+            pf[3 * k + 1] = species[s].p_f_sites[i][1];
+            pf[3 * k + 2] = species[s].p_f_sites[i][2];
+            qid[sid[k]] = chg[isite + i];   /* chg[] holds site_info[id].charge, accel.c:480-484 */
+         }
+         isite += species[s].nsites * species[s].nmols;
+      }
+   }
+   /* ewald() does not receive the potential table; an engine first set up
+      from it carries zero potentials until force_calc() supplies them */
+   for (i = 0; i < max_id * max_id; i++) {
+      if (potpar)
+         memcpy(pp + MOLDY_NPOTP * i, potpar[i].p, sizeof(double) * MOLDY_NPOTP);
+      else
+         memset(pp + MOLDY_NPOTP * i, 0, sizeof(double) * MOLDY_NPOTP);
+   }
+   d.nsites = system->nsites;
+   d.nmols = system->nmols;
+   d.nspecies = system->nspecies;
+   d.max_id = max_id;
+   d.ptype = system->ptype;
+   d.n_potpar = system->n_potpar;
+   d.spec_nmols = spn;
+   d.spec_nsites = sps;
+   d.spec_framework = spf;
+   d.spec_rdof = spr;
+   d.spec_site_id = sid;
+   d.spec_p_f_sites = pf;
+   d.site_charge = qid;
+   d.potpar = pp;
+   d.cutoff = control.cutoff;
+   d.subcell = control.subcell;
+   d.alpha = control.alpha;
+   d.k_cutoff = control.k_cutoff;
+   d.strict_cutoff = control.strict_cutoff;
+   d.molpbc = control.molpbc;
+   d.surface_dipole = control.surface_dipole;
+   rc = mgpu_set_system(eng, &d);
+   free(spn); free(sid); free(pf); free(qid); free(pp);
+   if (rc)
+      fatal_mgpu(rc);
+   prec = getenv("MOLDY_AMD_PRECISION");
+   mgpu_set_precision(eng, prec && !strcmp(prec, "mixed") ? MGPU_MIXED : MGPU_FP64);
+   sys_nsites = system->nsites;
+   sys_nmols = system->nmols;
+   sys_have_pot = potpar != NULL;
+}
+
+static void flat_h(system_mt *system, double h[9])
+{
+   int i, j;
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++)
+         h[3 * i + j] = system->h[i][j];
+}
+
+/* src/force.c:1108 */
+void force_calc(real **site, real **site_force, system_mt *system, spec_mt *species, real *chg,
+                pot_mt *potpar, double *pe, mat_mt stress)
+{
+   double h[9];
+   int rc, ld = (int)(site[1] - site[0]), k;
+   mgpu_fc_info info;
+   ensure_engine(system, species, chg, potpar);
+   flat_h(system, h);
+   rc = mgpu_force_calc(eng, h, site[0], ld, (const double *)system->c_of_m, chg, site_force[0],
+                        pe, &stress[0][0], &info);
+   if (rc)
+      fatal_mgpu(rc);
+   if (fc_init) {   /* src/force.c:1167 */
+      note("Intramolecular potential energy correction = %g", info.eintra * CONV_E);
+      fc_init = 0;
+   }
+   if (info.n_nabors != onabor) {   /* src/force.c:218-222, messages.h:91 */
+      note("Neighbour list contains %d cells", 2 * info.n_nabors);
+      onabor = info.n_nabors;
+   }
+   if (info.nx != onx || info.ny != ony || info.nz != onz) {   /* src/force.c:1240-1243 */
+      note("MD cell divided into %d subcells (%dx%dx%d)", info.nx * info.ny * info.nz, info.nx,
+           info.ny, info.nz);
+      onx = info.nx; ony = info.ny; onz = info.nz;
+   }
+   for (k = 0; k < info.n_too_close && k < 16; k++)   /* src/force.c:944-946 */
+      message(NULL, NULL, SEV_WARNING, "Sites %d and %d closer than %fA.", info.too_close_i[k],
+              info.too_close_j[k], sqrt(0.25));
+}
+
+/* src/ewald.c:280 */
+void ewald(real **site, real **site_force, system_mp system, spec_mt *species, real *chg,
+           double *pe, real (*stress)[3])
+{
+   double h[9];
+   int rc, ld = (int)(site[1] - site[0]);
+   mgpu_ew_info info;
+   ensure_engine(system, species, chg, NULL);
+   flat_h(system, h);
+   rc = mgpu_ewald(eng, h, site[0], ld, chg, site_force[0], pe, &stress[0][0], &info);
+   if (rc)
+      fatal_mgpu(rc);
+   if (ew_init) {   /* src/ewald.c:424, 463 */
+      note("Ewald self-energy = %f kJ/mol", info.self_energy * CONV_E);
+      note("%d K-vectors included in reciprocal-space sum", info.nhkl);
+      ew_init = 0;
+   }
+}
+
+/* poteval, src/force.c:632-650: the kernel energy of one pair at distance r
+ * with charge product chgsq (Coulomb term only when alpha > 0). */
+double poteval(real *p, double r, int ptype, double chgsq)
+{
+   const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741,
+                A4 = -1.453152027, A5 = 1.061405429;
+   double alpha = control.alpha, t = 0.0, r2r, r4, r6, r8, r12;
+   if (alpha > 0.0) {
+      double ar = alpha * r, tt = 1.0 / (1.0 + P * ar);
+      t = tt * (A1 + tt * (A2 + tt * (A3 + tt * (A4 + tt * A5)))) * chgsq * exp(-ar * ar) / r;
+   }
+   r2r = 1.0 / (r * r);
+   switch (ptype) {
+   case 0:
+      r6 = p[1] * p[1] * r2r; r6 = r6 * r6 * r6; r12 = r6 * r6;
+      return t + p[0] * (r12 - r6);
+   case 1:
+      return t - p[0] * r2r * r2r * r2r + p[1] * exp(-p[2] * r);
+   case 2:
+      return t + p[0] * exp(-p[1] * r) - p[2] * exp(-p[3] * r);
+   case 3:
+      r4 = r2r * r2r; r6 = r2r * r4; r8 = p[5] * r4 * r4; r12 = p[2] * r6 * r6;
+      return t + p[0] * exp(-p[1] * r) + r12 - p[3] * r4 - p[4] * r6 - r8;
+   case 4:
+      r4 = r2r * r2r; r6 = r2r * r4;
+      return t + p[0] * r4 + p[1] * r6 + p[2] * r6 * r6;
+   case 6:
+      return t + p[0] * exp((p[1] - r) * p[2]) - p[3] * r2r * r2r * r2r +
+             p[4] * exp(-2.0 * p[5] * (r - p[6])) - p[4] * 2.0 * exp(-p[5] * (r - p[6]));
+   default:
+      message(NULL, NULL, SEV_FATAL, "Potential type %d unknown", ptype);   /* UNKPTY */
+      return 0.0;
+   }
+}
+
+void moldy_amd_release(void)
+{
+   if (eng)
+      mgpu_destroy(eng);
+   eng = NULL;
+   sys_nsites = sys_nmols = -1;
+   sys_have_pot = 0;
+   onx = ony = onz = onabor = 0;
+   fc_init = ew_init = 1;
+}
