@@ -15,6 +15,8 @@
 // periodic image code 9*(sx+1)+3*(sy+1)+(sz+1).
 #include "mgpu_internal.h"
 
+#include <algorithm>
+
 using namespace mgpu;
 
 #define FLAG_BIN_RANGE 1
@@ -84,6 +86,50 @@ void mgpu::device_scan_int(mgpu_engine *e, const int *in, int *out, int n) {
    k_scan_block<<<nb, 1024, 0, e->stream>>>(in, out, bsum, n);
    k_scan_sums<<<1, 1024, 0, e->stream>>>(bsum, (n + 1023) / 1024, total);
    k_scan_add<<<nb, 1024, 0, e->stream>>>(out, bsum, n, total);
+}
+
+// ------------------------------------------------- column sums (two stages)
+// out[c] = sum over rows of in[r*ncols + c], ncols <= 16, summed in a fixed
+// order (per-block partials, then one block), so results are reproducible.
+static __global__ __launch_bounds__(256) void k_rc1(const double *__restrict__ in, int nrows, int ncols,
+                                                    int chunk, double *__restrict__ tmp) {
+   __shared__ double ws[4][16];
+   const int r0 = blockIdx.x * chunk, r1 = min(nrows, r0 + chunk);
+   double acc[16];
+#pragma unroll
+   for (int c = 0; c < 16; c++) acc[c] = 0.0;
+   for (int r = r0 + (int)threadIdx.x; r < r1; r += 256)
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+         if (c < ncols) acc[c] += in[(size_t)r * ncols + c];
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+   for (int c = 0; c < 16; c++) {
+      if (c >= ncols) break;
+      double t = wave_sum(acc[c]);
+      if (lane == 0) ws[w][c] = t;
+   }
+   __syncthreads();
+   if (threadIdx.x < ncols)
+      tmp[(size_t)blockIdx.x * ncols + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+static __global__ void k_rc2(const double *__restrict__ tmp, int nb, int ncols, double *__restrict__ out) {
+   if ((int)threadIdx.x < ncols) {
+      double s = 0.0;
+      for (int b = 0; b < nb; b++) s += tmp[(size_t)b * ncols + threadIdx.x];
+      out[threadIdx.x] = s;
+   }
+}
+
+void mgpu::reduce_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *out_dev) {
+   if (ncols > 16) throw StatusError{MGPU_E_ARG, "reduce_cols: more than 16 columns"};
+   int nb = std::max(1, std::min(512, (nrows + 2047) / 2048));
+   int chunk = (nrows + nb - 1) / nb;
+   e->d_rtmp.reserve((size_t)nb * ncols + 16);
+   k_rc1<<<nb, 256, 0, e->stream>>>(in, nrows, ncols, chunk, e->d_rtmp.p);
+   k_rc2<<<1, 64, 0, e->stream>>>(e->d_rtmp.p, nb, ncols, out_dev);
 }
 
 // ---------------------------------------------------------------- binning

@@ -577,7 +577,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cell_nsites); fr(e->d_cell_site_start); fr(e->d_scan_tmp); fr(e->d_corig); fr(e->d_cpos);
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
-   fr(e->d_groups); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
    for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
    if (e->stream) (void)hipStreamDestroy(e->stream);
    delete e;
@@ -899,7 +899,7 @@ __global__ void k_sum_cols_e(const double *__restrict__ in, int nrows, int ncols
 
 void mgpu::sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *host_out) {
    e->d_out.reserve(64);
-   k_sum_cols_e<<<ncols, 256, 0, e->stream>>>(in, nrows, ncols, e->d_out.p + 40);
+   reduce_cols(e, in, nrows, ncols, e->d_out.p + 40);
    HIPCHK(hipMemcpyAsync(host_out, e->d_out.p + 40, sizeof(double) * ncols, hipMemcpyDeviceToHost, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
 }

@@ -18,6 +18,8 @@
 // memory except the per-k weights (uniform loads).
 #include "mgpu_internal.h"
 
+#include <algorithm>
+
 using namespace mgpu;
 
 namespace {
@@ -214,22 +216,26 @@ struct EfArgs {
    int ns, nsp, nsxf, nrows;
    double c2;
    const KRow *rows;
+   const int *rpart;        // [npart+1] row ranges of the partitions
    const double *w;
    const double2 *tx, *ty, *tz, *tz1;
-   double *force;
+   double *fpart;           // [npart][3][nsp]
 };
 
-// Force back-projection, one thread per site (ewald.c:557-579).
+// Force back-projection (ewald.c:557-579): one thread per site, the rows
+// split over blockIdx.y so that enough waves are resident to hide the
+// latency of the l recurrence; partials are summed in a fixed order.
 __global__ __launch_bounds__(256) void k_ewald_force(EfArgs a) {
    int i = blockIdx.x * 256 + threadIdx.x;
    if (i >= a.ns) return;
+   const int part = blockIdx.y;
    const bool fw = i >= a.nsxf;
    const int wo = fw ? 2 : 0;
    const double2 z1 = a.tz1[i];
    double fx = 0, fy = 0, fz = 0;
    int hcur = -1;
    double2 xh = make_double2(1, 0);
-   for (int r = 0; r < a.nrows; r++) {
+   for (int r = a.rpart[part]; r < a.rpart[part + 1]; r++) {
       const KRow R = a.rows[r];
       if (R.h != hcur) {
          hcur = R.h;
@@ -256,9 +262,21 @@ __global__ __launch_bounds__(256) void k_ewald_force(EfArgs a) {
          z = cmul(z, z1);
       }
    }
-   a.force[i] += fx;
-   a.force[a.nsp + i] += fy;
-   a.force[2 * a.nsp + i] += fz;
+   double *fp = a.fpart + (size_t)part * 3 * a.nsp;
+   fp[i] = fx;
+   fp[a.nsp + i] = fy;
+   fp[2 * a.nsp + i] = fz;
+}
+
+__global__ void k_add_parts(int ns, int nsp, int npart, const double *__restrict__ fpart,
+                            double *__restrict__ force) {
+   int i = blockIdx.x * 256 + threadIdx.x;
+   if (i >= ns) return;
+   for (int c = 0; c < 3; c++) {
+      double s = 0.0;
+      for (int p = 0; p < npart; p++) s += fpart[((size_t)p * 3 + c) * nsp + i];
+      force[(size_t)c * nsp + i] += s;
+   }
 }
 
 __global__ void k_sum_cols(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
@@ -347,15 +365,31 @@ void mgpu::run_ewald_fp64(mgpu_engine *e, double *out7) {
       k_ewald_k<<<nkb, 256, 0, e->stream>>>(ek);
    }
    e->d_out.reserve(64);
-   k_sum_cols<<<7, 256, 0, e->stream>>>(e->d_red.p, nkb, 8, e->d_out.p + 8);
+   reduce_cols(e, e->d_red.p, nkb, 8, e->d_out.p + 8);
+   // row partitions with about equal numbers of k vectors
+   const int npart = std::max(1, std::min(16, e->nrows / 8));
+   std::vector<int> rp(npart + 1, e->nrows);
+   rp[0] = 0;
+   {
+      int p = 1;
+      for (int r = 0; r < e->nrows && p < npart; r++)
+         if ((long)e->rows[r].kstart * npart >= (long)p * e->nk) rp[p++] = r;
+      for (; p < npart; p++) rp[p] = e->nrows;
+   }
+   e->d_rpart.reserve(npart + 1);
+   HIPCHK(hipMemcpyAsync(e->d_rpart.p, rp.data(), sizeof(int) * (npart + 1), hipMemcpyHostToDevice, e->stream));
+   e->d_fpart.reserve((size_t)npart * 3 * nsp);
    EfArgs ef;
    ef.ns = e->nsites; ef.nsp = e->nsp; ef.nsxf = e->nsitesxf; ef.nrows = e->nrows;
-   ef.c2 = ek.c2; ef.rows = e->d_rows.p; ef.w = e->d_wk.p;
+   ef.c2 = ek.c2; ef.rows = e->d_rows.p; ef.rpart = e->d_rpart.p; ef.w = e->d_wk.p;
    ef.tx = e->d_tx.p; ef.ty = e->d_ty.p; ef.tz = e->d_tz.p; ef.tz1 = tz1;
-   ef.force = e->d_force_site.p;
+   ef.fpart = e->d_fpart.p;
    {
       ProfScope ps(e, "ewald_force");
-      k_ewald_force<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(ef);
+      dim3 g((e->nsites + 255) / 256, npart);
+      k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
+      k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p,
+                                                                  e->d_force_site.p);
    }
    HIPCHK(hipMemcpyAsync(out7, e->d_out.p + 8, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
 }

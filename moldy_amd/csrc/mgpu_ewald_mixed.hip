@@ -630,6 +630,6 @@ void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
       run(e->nsitesxf, ns, Wtn);
    }
    e->d_out.reserve(64);
-   k_mix_sum<<<7, 256, 0, e->stream>>>(bpart, nred, 8, e->d_out.p + 24);
+   reduce_cols(e, bpart, nred, 8, e->d_out.p + 24);
    HIPCHK(hipMemcpyAsync(out7, e->d_out.p + 24, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
 }

@@ -236,7 +236,7 @@ void mgpu::run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9
    double hd[3] = {0, 0, 0};
    if (e->alpha > 1e-7) {
       k_dipole<<<nb, 256, 0, e->stream>>>(ns, nsp, e->d_site.p, e->d_chg_site.p, e->d_red.p);
-      k_sum_cols2<<<3, 256, 0, e->stream>>>(e->d_red.p, nb, 3, out);
+      reduce_cols(e, e->d_red.p, nb, 3, out);
       HIPCHK(hipMemcpyAsync(hd, out, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
       if (e->surface_dipole) {
@@ -260,8 +260,8 @@ void mgpu::run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9
    k_vir_mols<<<nmxb, 256, 0, e->stream>>>(e->nmolsxf, e->h[0], e->h[1], e->h[2], e->h[3], e->h[4],
                                            e->h[5], e->h[6], e->h[7], e->h[8], e->d_cm.p, e->d_mforce.p,
                                            mpart);
-   k_sum_cols2<<<9, 256, 0, e->stream>>>(e->d_red.p, nb, 9, out);
-   k_sum_cols2<<<9, 256, 0, e->stream>>>(mpart, nmxb, 9, out + 9);
+   reduce_cols(e, e->d_red.p, nb, 9, out);
+   reduce_cols(e, mpart, nmxb, 9, out + 9);
    double corr[18];
    HIPCHK(hipMemcpyAsync(corr, out, 18 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));

@@ -109,6 +109,7 @@ struct mgpu_engine {
    mgpu::DBuf<double> d_mforce, d_mtorque;     // [nmols][3], [nmols_r][3]
    mgpu::DBuf<double> d_red;                   // reduction scratch
    mgpu::DBuf<double> d_out;                   // small result block
+   mgpu::DBuf<double> d_rtmp;                  // column-sum partials
    mgpu::DBuf<int> d_iflags;                   // status flags / counters
 
    // ---- cells ----
@@ -140,6 +141,8 @@ struct mgpu_engine {
    mgpu::DBuf<double> d_sk;                     // S(k) partial sums
    mgpu::DBuf<int4> d_groups;                   // S(k) workgroup -> k rows
    mgpu::DBuf<double> d_wk;                     // per-k weights
+   mgpu::DBuf<int> d_rpart;                     // row partitions of the force pass
+   mgpu::DBuf<double> d_fpart;                  // per-partition site forces
    int sk_chunks = 0;
 
    // ---- mixed-precision ewald ----
@@ -171,6 +174,7 @@ void run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9], dou
               const double *vir_real, const double *st_recip);
 void device_scan_int(mgpu_engine *e, const int *in, int *out, int n);  // exclusive, out[n]=total
 void sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *host_out);
+void reduce_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *out_dev);
 double host_det(const double a[9]);
 void host_invert(const double a[9], double b[9]);
 

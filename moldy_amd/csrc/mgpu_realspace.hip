@@ -358,6 +358,6 @@ void mgpu::run_realspace(mgpu_engine *e, double *out7) {
       }
    }
    e->d_out.reserve(64);
-   k_reduce_rows<<<7, 256, 0, e->stream>>>(e->d_cell_part.p, nc, 8, e->d_out.p);
+   reduce_cols(e, e->d_cell_part.p, nc, 8, e->d_out.p);
    HIPCHK(hipMemcpyAsync(out7, e->d_out.p, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
 }
