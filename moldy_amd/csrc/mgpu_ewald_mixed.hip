@@ -105,7 +105,10 @@ struct SkArgs {
 template <int NB>
 __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
    constexpr int NCOL = 16 * NB;
-   __shared__ __attribute__((aligned(16))) half_t sA[4][FROWS][KS + 8];
+   constexpr int BITEMS = 4 * NCOL * (KS / 8);   // (plane, column, 8-site group) half8 items
+   constexpr int BPER = (BITEMS + 255) / 256;
+   __shared__ __attribute__((aligned(16))) half_t sA[4][FROWS][KS + 8];   // Ar hi/lo, Ai hi/lo
+   __shared__ __attribute__((aligned(16))) half_t sB[4][NCOL][KS + 8];    // Zr hi/lo, Zi hi/lo
    __shared__ int2 shk[FROWS];
    const int tile = blockIdx.x, slice = blockIdx.y;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -117,58 +120,87 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
    for (int m = 0; m < 4; m++)
 #pragma unroll
       for (int n = 0; n < NB; n++) acc[m][n] = floatx4{0, 0, 0, 0};
-   const half_t *Zh = wn == 0 ? a.Zhr : a.Zhi;
-   const half_t *Zl = wn == 0 ? a.Zlr : a.Zli;
-   const int si = tid & 31, rg = tid >> 5;
-   for (int i0 = sl.x; i0 < sl.y; i0 += KS) {
+   __syncthreads();
+   // A generation: one row, eight consecutive sites per thread
+   const int rr = tid >> 2, sg = tid & 3;
+   const int2 hk = shk[rr];
+   const bool rvalid = hk.x >= 0;
+   const size_t xoff = rvalid ? (size_t)hk.x * a.nsp : 0;
+   const size_t yoff = rvalid ? (size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp : 0;
+   const float ysgn = hk.y < 0 ? -1.f : 1.f;
+   float4 xr[4], yr[4];
+   half8 zr[BPER];
+   // global loads of one 32-site step into registers (issued a step ahead)
+   auto load_step = [&](int i0) {
+      const int ib = i0 + 8 * sg;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+         xr[v] = rvalid ? *(const float4 *)(a.X + xoff + ib + 2 * v) : float4{0, 0, 0, 0};
+         yr[v] = rvalid ? *(const float4 *)(a.Y + yoff + ib + 2 * v) : float4{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int j = 0; j < BPER; j++) {
+         const int q = tid + 256 * j;
+         half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
+         if (q < BITEMS) {
+            const int p = q / (NCOL * 4), rem = q - p * (NCOL * 4), c = rem >> 2, gg = rem & 3;
+            const int l = c - a.lmax, al = l < 0 ? -l : l;
+            if (al <= a.lmax) {
+               const half_t *base = p == 0 ? a.Zhr : (p == 1 ? a.Zlr : (p == 2 ? a.Zhi : a.Zli));
+               v = *(const half8 *)(base + (size_t)al * a.nsp + i0 + 8 * gg);
+               if (p >= 2 && l < 0) v = -v;   // exp(-i l c*.r) = conj
+            }
+         }
+         zr[j] = v;
+      }
+   };
+   int i0 = sl.x;
+   if (i0 < sl.y) load_step(i0);
+   for (; i0 < sl.y; i0 += KS) {
       __syncthreads();
       {
-         const int i = i0 + si;
-         const bool valid = i >= sl.z && i < sl.w;
+         const int ib = i0 + 8 * sg;
+         half8 arh, arl, aih, ail;
 #pragma unroll
-         for (int j = 0; j < 8; j++) {
-            const int rr = rg + 8 * j;
-            const int2 hk = shk[rr];
-            float ar = 0.f, ai = 0.f;
-            if (hk.x >= 0 && valid) {
-               float2 x = a.X[(size_t)hk.x * a.nsp + i];
-               float2 y = a.Y[(size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp + i];
-               if (hk.y < 0) y.y = -y.y;
-               ar = x.x * y.x - x.y * y.y;
-               ai = x.y * y.x + x.x * y.y;
+         for (int t = 0; t < 8; t++) {
+            const float4 xv = xr[t >> 1], yv = yr[t >> 1];
+            const float x0 = (t & 1) ? xv.z : xv.x, x1 = (t & 1) ? xv.w : xv.y;
+            const float y0 = (t & 1) ? yv.z : yv.x, y1 = ysgn * ((t & 1) ? yv.w : yv.y);
+            const bool ok = rvalid && ib + t >= sl.z && ib + t < sl.w;
+            const float ar = ok ? x0 * y0 - x1 * y1 : 0.f;
+            const float ai = ok ? x1 * y0 + x0 * y1 : 0.f;
+            half_t h, l;
+            split2f(ar, h, l);
+            arh[t] = h; arl[t] = l;
+            split2f(ai, h, l);
+            aih[t] = h; ail[t] = l;
+         }
+         *(half8 *)&sA[0][rr][8 * sg] = arh;
+         *(half8 *)&sA[1][rr][8 * sg] = arl;
+         *(half8 *)&sA[2][rr][8 * sg] = aih;
+         *(half8 *)&sA[3][rr][8 * sg] = ail;
+#pragma unroll
+         for (int j = 0; j < BPER; j++) {
+            const int q = tid + 256 * j;
+            if (q < BITEMS) {
+               const int p = q / (NCOL * 4), rem = q - p * (NCOL * 4), c = rem >> 2, gg = rem & 3;
+               *(half8 *)&sB[p][c][8 * gg] = zr[j];
             }
-            split2f(ar, sA[0][rr][si], sA[1][rr][si]);
-            split2f(ai, sA[2][rr][si], sA[3][rr][si]);
          }
       }
+      if (i0 + KS < sl.y) load_step(i0 + KS);
       __syncthreads();
-      half8 bh[NB], bl[NB];
 #pragma unroll
       for (int n = 0; n < NB; n++) {
-         const int l = n * 16 + lc - a.lmax;
-         const int al = l < 0 ? -l : l;
-         if (al <= a.lmax) {
-            const size_t o = (size_t)al * a.nsp + i0 + 8 * g;
-            bh[n] = *(const half8 *)(Zh + o);
-            bl[n] = *(const half8 *)(Zl + o);
-            if (wn == 1 && l < 0) {
-               bh[n] = -bh[n];
-               bl[n] = -bl[n];
-            }
-         } else {
-            bh[n] = half8{0, 0, 0, 0, 0, 0, 0, 0};
-            bl[n] = bh[n];
-         }
-      }
+         const half8 bh = *(const half8 *)&sB[2 * wn][n * 16 + lc][8 * g];
+         const half8 bl = *(const half8 *)&sB[2 * wn + 1][n * 16 + lc][8 * g];
 #pragma unroll
-      for (int m = 0; m < 4; m++) {
-         const half8 ah = *(const half8 *)&sA[2 * wm][m * 16 + lc][8 * g];
-         const half8 al8 = *(const half8 *)&sA[2 * wm + 1][m * 16 + lc][8 * g];
-#pragma unroll
-         for (int n = 0; n < NB; n++) {
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[n], acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[n], acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8, bh[n], acc[m][n], 0, 0, 0);
+         for (int m = 0; m < 4; m++) {
+            const half8 ah = *(const half8 *)&sA[2 * wm][m * 16 + lc][8 * g];
+            const half8 al8 = *(const half8 *)&sA[2 * wm + 1][m * 16 + lc][8 * g];
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8, bh, acc[m][n], 0, 0, 0);
          }
       }
    }
@@ -316,14 +348,36 @@ struct FArgs {
 template <int NB>
 __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    constexpr int NCOL = 16 * NB, K2 = 2 * NCOL;
-   __shared__ __attribute__((aligned(16))) half_t sW[4][2][16][K2 + 8];
-   extern __shared__ float2 sY[];   // [(kmax+1)][BSITES]
+   constexpr int CH = 4 * 2 * 16 * (K2 / 8);   // half8 chunks in one weight tile
+   constexpr int CPER = (CH + 511) / 512;
+   // at NB = 6 the prefetch registers would spill: load the next tile after the epilogue
+   constexpr bool EARLY = NB <= 5;
+   __shared__ __attribute__((aligned(16))) half_t sW[2][4][2][16][K2 + 8];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
    const int base = a.s0 + blockIdx.x * BSITES;
-   for (int q = tid; q < (a.kmax + 1) * BSITES; q += 512) {
-      const int k = q / BSITES, t = q - k * BSITES, i = base + t;
-      sY[q] = i < a.s1 ? a.Y[(size_t)k * a.nsp + i] : make_float2(0, 0);
-   }
+   const int ntile = a.nrows16 / 16;
+   half8 wr[CPER];
+   auto wload = [&](int rt) {
+#pragma unroll
+      for (int j = 0; j < CPER; j++) {
+         const int q = tid + 512 * j;
+         if (q < CH) {
+            const int c8 = q % (K2 / 8), row = (q / (K2 / 8)) % 16, kh = q / (16 * (K2 / 8));
+            wr[j] = *(const half8 *)(a.Wt + ((size_t)kh * a.nrows16 + rt * 16 + row) * K2 + 8 * c8);
+         }
+      }
+   };
+   auto wstore = [&](int buf) {
+#pragma unroll
+      for (int j = 0; j < CPER; j++) {
+         const int q = tid + 512 * j;
+         if (q < CH) {
+            const int c8 = q % (K2 / 8), row = (q / (K2 / 8)) % 16, kh = q / (16 * (K2 / 8));
+            *(half8 *)&sW[buf][kh >> 1][kh & 1][row][8 * c8] = wr[j];
+         }
+      }
+   };
+   wload(0);
    // Z operand [site][Zr | Zi over the padded l range], held for the whole kernel
    half8 zh[2][NB], zl[2][NB];
 #pragma unroll
@@ -350,24 +404,26 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    }
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
    const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
+   __shared__ float2 sY16[BSITES];   // exp(16 i b*.r) per site of the block
    float f[2][4][3];
+   float2 av[2][4];   // exp(i k.r) of the lane's current row, per site
 #pragma unroll
    for (int m = 0; m < 2; m++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
-   int cur_h = -1;
-   float2 xv[2][4];
-   __syncthreads();
-   const int ntile = a.nrows16 / 16;
-   for (int rt = 0; rt < ntile; rt++) {
-      // stage the weight tile: 4 kinds x hi/lo x 16 rows x K2
-      constexpr int CH = 4 * 2 * 16 * (K2 / 8);
-      for (int q = tid; q < CH; q += 512) {
-         const int c8 = q % (K2 / 8), row = (q / (K2 / 8)) % 16, kh = q / (16 * (K2 / 8));
-         *(half8 *)&sW[kh >> 1][kh & 1][row][8 * c8] =
-             *(const half8 *)(a.Wt + ((size_t)kh * a.nrows16 + rt * 16 + row) * K2 + 8 * c8);
+      for (int j = 0; j < 4; j++) {
+         f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
+         av[m][j] = make_float2(0, 0);
       }
-      __syncthreads();
+   if (tid < BSITES) {
+      const int i = base + tid;
+      sY16[tid] = (a.kmax >= 16 && i < a.s1) ? a.Y[(size_t)16 * a.nsp + i] : make_float2(0, 0);
+   }
+   int cur_h = -1, cur_k = -1000000;
+   wstore(0);
+   __syncthreads();
+   for (int rt = 0; rt < ntile; rt++) {
+      const int buf = rt & 1;
+      if (EARLY && rt + 1 < ntile) wload(rt + 1);
       floatx4 acc[2][4];
 #pragma unroll
       for (int m = 0; m < 2; m++)
@@ -377,8 +433,8 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
       for (int ks = 0; ks < NB; ks++) {
 #pragma unroll
          for (int kd = 0; kd < 4; kd++) {
-            const half8 bh = *(const half8 *)&sW[kd][0][lc][ks * 32 + 8 * g];
-            const half8 bl = *(const half8 *)&sW[kd][1][lc][ks * 32 + 8 * g];
+            const half8 bh = *(const half8 *)&sW[buf][kd][0][lc][ks * 32 + 8 * g];
+            const half8 bl = *(const half8 *)&sW[buf][kd][1][lc][ks * 32 + 8 * g];
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
@@ -391,27 +447,28 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
       if (r < a.nrows) {
          const int2 hk = a.rowhk[r];
          const float4 kv = a.rowk[r];
-         if (hk.x != cur_h) {
-            cur_h = hk.x;
-#pragma unroll
-            for (int m = 0; m < 2; m++)
-#pragma unroll
-               for (int j = 0; j < 4; j++) {
-                  const int i = base + 32 * wave + 16 * m + 4 * g + j;
-                  xv[m][j] = i < a.s1 ? a.X[(size_t)hk.x * a.nsp + i] : make_float2(0, 0);
-               }
-         }
+         const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
          const int ak = hk.y < 0 ? -hk.y : hk.y;
          const float ysg = hk.y < 0 ? -1.f : 1.f;
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-               const int t = 32 * wave + 16 * m + 4 * g + j;
-               float2 y = sY[ak * BSITES + t];
-               y.y *= ysg;
-               const float2 x = xv[m][j];
-               const float Ar = x.x * y.x - x.y * y.y, Ai = x.y * y.x + x.x * y.y;
+               const int i = base + 32 * wave + 16 * m + 4 * g + j;
+               float2 A;
+               if (step16) {
+                  // exp(i (k+16) b*.r) = exp(i k b*.r) exp(i 16 b*.r)
+                  const float2 y = av[m][j], s = sY16[i - base];
+                  A = make_float2(y.x * s.x - y.y * s.y, y.y * s.x + y.x * s.y);
+               } else {
+                  const bool in = i < a.s1;
+                  const float2 x = in ? a.X[(size_t)hk.x * a.nsp + i] : make_float2(0, 0);
+                  float2 y = in ? a.Y[(size_t)ak * a.nsp + i] : make_float2(0, 0);
+                  y.y *= ysg;
+                  A = make_float2(x.x * y.x - x.y * y.y, x.y * y.x + x.x * y.y);
+               }
+               av[m][j] = A;
+               const float Ar = A.x, Ai = A.y;
                const float Pr = acc[m][0][j] * inv_pw, Pi = acc[m][1][j] * inv_pw;
                const float Qr = acc[m][2][j] * inv_qw, Qi = acc[m][3][j] * inv_qw;
                const float t1 = Ar * Pi + Ai * Pr;
@@ -420,6 +477,12 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
                f[m][j][1] += kv.y * t1;
                f[m][j][2] += kv.z * t1 + a.c2 * t2;
             }
+         cur_h = hk.x;
+         cur_k = hk.y;
+      }
+      if (rt + 1 < ntile) {
+         if (!EARLY) wload(rt + 1);
+         wstore(buf ^ 1);
       }
       __syncthreads();
    }
@@ -470,14 +533,8 @@ __global__ void k_mix_sum(const double *__restrict__ in, int nrows, int ncols, d
 template <int NB> void launch_sk(dim3 g, hipStream_t s, const SkArgs &a) {
    k_mix_sk<NB><<<g, 256, 0, s>>>(a);
 }
-template <int NB> void launch_force(dim3 g, size_t shm, hipStream_t s, const FArgs &a) {
-   static bool attr = false;
-   if (!attr) {
-      HIPCHK(hipFuncSetAttribute((const void *)k_mix_force<NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 96 * 1024));
-      attr = true;
-   }
-   k_mix_force<NB><<<g, 512, shm, s>>>(a);
+template <int NB> void launch_force(dim3 g, size_t, hipStream_t s, const FArgs &a) {
+   k_mix_force<NB><<<g, 512, 0, s>>>(a);
 }
 
 #define NB_SWITCH(nb, CALL)                                                     \
@@ -499,8 +556,6 @@ void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
    const int ns = e->nsites, nsp = e->nsp, lmax = e->lmax;
    const int NB = (2 * lmax + 1 + 15) / 16;
    if (NB > 6) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (lmax > 47)"};
-   if ((e->kmax + 1) * BSITES * sizeof(float2) > 96 * 1024)
-      throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (kmax > 47)"};
    const int NCOL = 16 * NB, K2 = 2 * NCOL;
    const int nrows = e->nrows;
    const int ntile = (nrows + FROWS - 1) / FROWS;
@@ -617,7 +672,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
       f.rowhk = rowhk; f.rowk = rowk; f.X = X; f.Y = Y;
       f.Zhr = Zhr; f.Zlr = Zlr; f.Zhi = Zhi; f.Zli = Zli; f.maxbits = maxbits;
       f.force = e->d_force_site.p;
-      const size_t shm = sizeof(float2) * (e->kmax + 1) * BSITES;
+      const size_t shm = 0;
       auto run = [&](int s0, int s1, const half_t *wt) {
          if (s1 <= s0) return;
          f.s0 = s0; f.s1 = s1; f.Wt = wt;
