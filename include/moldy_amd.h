@@ -30,7 +30,9 @@ enum {
    MGPU_E_MULTIFW = 7,    /* multiple frameworks,              force.c:1219   */
    MGPU_E_PTYPE = 8,      /* UNKPTY,                           kernel.c:186   */
    MGPU_E_STATE = 9,      /* call order (no system set, ...)                  */
-   MGPU_E_CAPACITY = 10   /* a fixed-capacity limit of this engine exceeded   */
+   MGPU_E_CAPACITY = 10,  /* a fixed-capacity limit of this engine exceeded   */
+   MGPU_E_QNORM = 11      /* QNORM2 fatal, quaternion norm off by > 1e-4,
+                             src/leapfrog.c:104-106                            */
 };
 
 enum { MGPU_FP64 = 0, MGPU_MIXED = 1 };

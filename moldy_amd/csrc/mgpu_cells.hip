@@ -19,8 +19,6 @@
 
 using namespace mgpu;
 
-#define FLAG_BIN_RANGE 1
-#define FLAG_BIN_ROUND 2
 
 // ----------------------------------------------------------------- scan
 static __global__ void k_scan_block(const int *__restrict__ in, int *__restrict__ out,
@@ -259,8 +257,8 @@ void mgpu::run_binning(mgpu_engine *e, bool) {
    e->d_cposf.reserve(e->nsites + 1);
    e->d_cmeta.reserve(e->nsites + 1);
    e->d_corig.reserve(e->nsites + 1);
-   HIPCHK(hipMemsetAsync(e->d_cell_count.p, 0, sizeof(int) * (nc + 1), e->stream));
-   HIPCHK(hipMemsetAsync(e->d_cell_nsites.p, 0, sizeof(int) * (nc + 1), e->stream));
+   zero_async(e, e->d_cell_count.p, sizeof(int) * (nc + 1));
+   zero_async(e, e->d_cell_nsites.p, sizeof(int) * (nc + 1));
    BinArgs a;
    for (int i = 0; i < 9; i++) a.hi[i] = e->hinv[i];
    a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.ne = e->nentries; a.nsp = e->nsp;

@@ -237,6 +237,7 @@ void set_geometry(mgpu_engine *e, const double h[9]) {
       HIPCHK(hipMemcpyAsync(e->d_off.p, full.data(), sizeof(int4) * full.size(), hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipMemcpyAsync(e->d_offD.p, D.data(), sizeof(float4) * D.size(), hipMemcpyHostToDevice, e->stream));
       memcpy(e->off_key, key, sizeof key);
+      e->geom_version++;
    }
    {
       // smallest plane spacing 1/|row of h^-1|; cell diagonal bounds how far
@@ -307,6 +308,7 @@ void set_geometry(mgpu_engine *e, const double h[9]) {
                                e->stream));
          HIPCHK(hipStreamSynchronize(e->stream));
          memcpy(e->kset_key, kkey, sizeof kkey);
+         e->geom_version++;
       }
    }
 }
@@ -433,42 +435,62 @@ void download_forces(mgpu_engine *e, double *f, int ld) {
                             hipMemcpyDeviceToHost, e->stream));
 }
 
-void check_flags(mgpu_engine *e, int *tooclose_out) {
-   int fl[34];
-   HIPCHK(hipMemcpyAsync(fl, e->d_iflags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
-   HIPCHK(hipStreamSynchronize(e->stream));
-   if (tooclose_out) memcpy(tooclose_out, fl + 1, sizeof(int) * 33);
-   if (fl[0] & 1) throw StatusError{MGPU_E_BIN_RANGE, "Co-ordinate out of range in BIN (fill_cells)"};
-   if (fl[0] & 2) throw StatusError{MGPU_E_BIN_ROUND, "Rounding problem in BIN (fill_cells)"};
-}
-
-// Real-space pass on the sites in e->d_site; forces accumulate into
-// e->d_force_site.  Returns pe (incl. -eintra) and the 6 virial terms.
-void real_pass(mgpu_engine *e, double *pe, double vir6[6], bool pre_virial, mgpu_fc_info *info) {
-   HIPCHK(hipMemsetAsync(e->d_iflags.p, 0, sizeof(int) * 40, e->stream));
-   run_binning(e, false);
-   double v7[7];
-   run_realspace(e, v7);
-   int tc[33];
-   check_flags(e, tc);
-   if (e->max_id > MGPU_MAX_ID) throw StatusError{MGPU_E_CAPACITY, "max_id > 16"};
-   if (e->precision == MGPU_MIXED && e->intra_exact) *pe += v7[0];
-   else *pe += v7[0] - e->eintra;
-   for (int q = 0; q < 6; q++) vir6[q] = v7[1 + q];
-   (void)pre_virial;
+// Bin/normalisation errors and too-close pairs from the flags block (host copy).
+void handle_flags(mgpu_engine *e, const int *fl) {
+   if (fl[0] & FLAG_BIN_RANGE) throw StatusError{MGPU_E_BIN_RANGE, "Co-ordinate out of range in BIN (fill_cells)"};
+   if (fl[0] & FLAG_BIN_ROUND) throw StatusError{MGPU_E_BIN_ROUND, "Rounding problem in BIN (fill_cells)"};
+   if (fl[0] & FLAG_QNORM)
+      throw StatusError{MGPU_E_QNORM, "quaternion normalisation failed (leapfrog.c normalise) at molecule " +
+                                          std::to_string(fl[40])};
    mgpu_fc_info fi{};
    fi.nx = e->nx; fi.ny = e->ny; fi.nz = e->nz; fi.n_nabors = e->n_half;
-   fi.n_too_close = tc[0];
-   for (int q = 0; q < 16 && q < tc[0]; q++) { fi.too_close_i[q] = tc[1 + 2 * q]; fi.too_close_j[q] = tc[2 + 2 * q]; }
+   fi.n_too_close = fl[1];
+   for (int q = 0; q < 16 && q < fl[1]; q++) { fi.too_close_i[q] = fl[2 + 2 * q]; fi.too_close_j[q] = fl[3 + 2 * q]; }
    fi.eintra = e->eintra;
    e->last_fc = fi;
-   if (info) *info = fi;
+}
+
+void sync_flags(mgpu_engine *e) {
+   int fl[41];
+   HIPCHK(hipMemcpyAsync(fl, e->d_iflags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   handle_flags(e, fl);
+}
+
+// Real-space pass on the sites in e->d_site: binning and the pair kernel;
+// forces accumulate into e->d_force_site, sums into d_out[OUT_REAL..].
+void real_pass_device(mgpu_engine *e, bool clear_flags) {
+   if (e->max_id > MGPU_MAX_ID) throw StatusError{MGPU_E_CAPACITY, "max_id > 16"};
+   if (clear_flags) zero_async(e, e->d_iflags.p, sizeof(int) * 40);
+   run_binning(e, false);
+   run_realspace(e);
+}
+
+// Host-buffer form of the real-space pass (force_calc): pe (incl. -eintra)
+// and the 6 virial terms.
+void real_pass(mgpu_engine *e, double *pe, double vir6[6], mgpu_fc_info *info) {
+   real_pass_device(e, true);
+   double v7[7];
+   int fl[41];
+   HIPCHK(hipMemcpyAsync(v7, e->d_out.p + OUT_REAL, sizeof v7, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipMemcpyAsync(fl, e->d_iflags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   handle_flags(e, fl);
+   // the mixed kernel leaves intramolecular pairs out when each is listed once
+   *pe += (e->precision == MGPU_MIXED && e->intra_exact) ? v7[0] : v7[0] - e->eintra;
+   for (int q = 0; q < 6; q++) vir6[q] = v7[1 + q];
+   if (info) *info = e->last_fc;
+}
+
+void recip_pass_device(mgpu_engine *e) {
+   if (e->precision == MGPU_MIXED) run_ewald_mixed(e);
+   else run_ewald_fp64(e);
 }
 
 void recip_pass(mgpu_engine *e, double *pe, double st6[6], mgpu_ew_info *info) {
+   recip_pass_device(e);
    double o7[7];
-   if (e->precision == MGPU_MIXED) run_ewald_mixed(e, o7);
-   else run_ewald_fp64(e, o7);
+   HIPCHK(hipMemcpyAsync(o7, e->d_out.p + OUT_RECIP, sizeof o7, hipMemcpyDeviceToHost, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
    double vol = host_det(e->h);
    *pe += o7[0] - e->self_energy + e->sheet_energy / vol;
@@ -483,6 +505,20 @@ void recip_pass(mgpu_engine *e, double *pe, double st6[6], mgpu_ew_info *info) {
    if (info) *info = wi;
 }
 
+}  // namespace
+
+// eval_forces on the device (src/accel.c:398-617) from d_cm/d_quat: site
+// forces, molecular forces/torques and d_out[OUT_RES..]; no host
+// synchronisation, so a caller may capture it.
+void mgpu::eval_forces_device(mgpu_engine *e, bool clear_flags) {
+   zero_async(e, e->d_force_site.p, sizeof(double) * 3 * e->nsp);
+   launch_make_sites(e, e->molpbc ? 1 : 0);
+   real_pass_device(e, clear_flags);
+   if (e->alpha > 1e-7) recip_pass_device(e);
+   run_glue(e);
+}
+
+namespace {
 }  // namespace
 
 // Pre-existing site_force contributes sum r_i F_i to the reference's virial
@@ -674,6 +710,7 @@ int mgpu_set_system(mgpu_engine *e, const mgpu_system *s) {
       e->d_mforce.reserve(3 * (size_t)e->nmols + 3);
       e->d_mtorque.reserve(3 * (size_t)std::max(e->nmols_r, 1));
       compute_statics(e, chg.data());
+      e->geom_version++;
       e->n_full = 0;
       e->nk = 0;
       e->have_system = true;
@@ -700,12 +737,12 @@ int mgpu_force_calc(mgpu_engine *e, const double h[9], const double *site, int l
       // pre-existing forces' share of the virial
       int nb = (e->nsites + 255) / 256;
       e->d_red.reserve((size_t)nb * 6 + 16);
-      e->d_out.reserve(64);
+      e->d_out.reserve(OUT_SIZE);
       k_pre_virial<<<nb, 256, 0, e->stream>>>(e->nsites, e->nsp, e->d_site.p, e->d_force_site.p, e->d_red.p);
       double pv[6];
       sum_cols(e, e->d_red.p, nb, 6, pv);
       double v6[6], pe_loc = 0.0;
-      real_pass(e, &pe_loc, v6, true, info);
+      real_pass(e, &pe_loc, v6, info);
       download_forces(e, site_force, ld);
       HIPCHK(hipStreamSynchronize(e->stream));
       *pe += pe_loc;
@@ -755,14 +792,20 @@ int mgpu_eval_forces(mgpu_engine *e, double pe[2], double dip[3], double stress[
    return guarded(e, [&] {
       if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
       HIPCHK(hipSetDevice(e->device));
-      HIPCHK(hipMemsetAsync(e->d_force_site.p, 0, sizeof(double) * 3 * e->nsp, e->stream));
-      launch_make_sites(e, e->molpbc ? 1 : 0);
-      double per = 0.0, pek = 0.0, v6[6] = {0}, st6[6] = {0};
-      real_pass(e, &per, v6, false, nullptr);
-      if (e->alpha > 1e-7) recip_pass(e, &pek, st6, nullptr);
-      double vol = host_det(e->h);
-      run_glue(e, pe, dip, stress, pek, v6, st6);
-      pe[0] = per + e->dist / vol;
+      eval_forces_device(e, true);
+      double r[14];
+      int fl[41];
+      HIPCHK(hipMemcpyAsync(r, e->d_out.p + OUT_RES, sizeof r, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(fl, e->d_iflags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      handle_flags(e, fl);
+      if (pe) { pe[0] = r[0]; pe[1] = r[1]; }
+      if (dip) for (int c = 0; c < 3; c++) dip[c] = r[2 + c];
+      if (stress) for (int q = 0; q < 9; q++) stress[q] = r[5 + q];
+      mgpu_ew_info wi{};
+      wi.hmax = e->hmax; wi.kmax = e->kmax; wi.lmax = e->lmax; wi.nhkl = e->nk; wi.nsitesxf = e->nsitesxf;
+      wi.self_energy = e->self_energy; wi.sheet_energy = e->sheet_energy;
+      e->last_ew = wi;
       return 0;
    });
 }
@@ -801,7 +844,7 @@ int mgpu_cells(mgpu_engine *e, const double h[9], const double *site, int ld, co
       upload_sites(e, site, ld);
       if (e->molpbc && cm)
          HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
-      HIPCHK(hipMemsetAsync(e->d_iflags.p, 0, sizeof(int) * 40, e->stream));
+      zero_async(e, e->d_iflags.p, sizeof(int) * 40);
       run_binning(e, false);
       if (dims3) { dims3[0] = e->nx; dims3[1] = e->ny; dims3[2] = e->nz; }
       if (entry_cell)
@@ -813,7 +856,7 @@ int mgpu_cells(mgpu_engine *e, const double h[9], const double *site, int ld, co
       if (cell_entries)
          HIPCHK(hipMemcpyAsync(cell_entries, e->d_cell_entries.p, sizeof(int) * e->nentries, hipMemcpyDeviceToHost,
                                e->stream));
-      check_flags(e, nullptr);
+      sync_flags(e);
       return 0;
    });
 }
@@ -828,9 +871,9 @@ int mgpu_pairlist(mgpu_engine *e, const double h[9], const double *site, int ld,
       upload_sites(e, site, ld);
       if (e->molpbc && cm)
          HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
-      HIPCHK(hipMemsetAsync(e->d_iflags.p, 0, sizeof(int) * 40, e->stream));
+      zero_async(e, e->d_iflags.p, sizeof(int) * 40);
       run_binning(e, false);
-      check_flags(e, nullptr);
+      sync_flags(e);
       std::vector<int> off;
       int n = neighbour_offsets(h, cutoff, e->nx, e->ny, e->nz, strict, off);
       if (n < 0) return fail(e, -n, "cutoff radius exceeds the image-cell range (CUTOFF)");
@@ -882,24 +925,9 @@ int mgpu_profile_get(const mgpu_engine *ec, const char *name, double *avg_ms, lo
 
 }  // extern "C"
 
-__global__ void k_sum_cols_e(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
-   __shared__ double ws[16];
-   int col = blockIdx.x;
-   double s = 0;
-   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
-   s = wave_sum(s);
-   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      double t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
-      out[col] = t;
-   }
-}
-
 void mgpu::sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *host_out) {
-   e->d_out.reserve(64);
-   reduce_cols(e, in, nrows, ncols, e->d_out.p + 40);
-   HIPCHK(hipMemcpyAsync(host_out, e->d_out.p + 40, sizeof(double) * ncols, hipMemcpyDeviceToHost, e->stream));
+   e->d_out.reserve(OUT_SIZE);
+   reduce_cols(e, in, nrows, ncols, e->d_out.p + OUT_TMP);
+   HIPCHK(hipMemcpyAsync(host_out, e->d_out.p + OUT_TMP, sizeof(double) * ncols, hipMemcpyDeviceToHost, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
 }

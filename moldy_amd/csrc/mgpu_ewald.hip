@@ -279,20 +279,6 @@ __global__ void k_add_parts(int ns, int nsp, int npart, const double *__restrict
    }
 }
 
-__global__ void k_sum_cols(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
-   __shared__ double ws[16];
-   int col = blockIdx.x;
-   double s = 0;
-   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
-   s = wave_sum(s);
-   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      double t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
-      out[col] = t;
-   }
-}
 
 }  // namespace
 
@@ -313,13 +299,15 @@ static void ewald_tables(mgpu_engine *e) {
        e->d_tz.p + (size_t)(e->lmax + 1) * nsp);
 }
 
-void mgpu::run_ewald_fp64(mgpu_engine *e, double *out7) {
+void mgpu::run_ewald_fp64(mgpu_engine *e) {
    ewald_tables(e);
    const size_t nsp = e->nsp;
    const double2 *tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * nsp;
-   // k groups: <= 256 k vectors, <= KROWS rows each (host-side, cached with the k set)
-   std::vector<SkGroup> groups;
-   {
+   // k groups: <= 256 k vectors, <= KROWS rows each, and row partitions of
+   // the force pass with about equal numbers of k vectors; both are host
+   // tables uploaded when the k set changes
+   if (e->ew_plan_version != e->geom_version) {
+      std::vector<SkGroup> groups;
       int r = 0;
       while (r < e->nrows) {
          SkGroup g{r, 0, e->rows[r].kstart, 0};
@@ -330,11 +318,24 @@ void mgpu::run_ewald_fp64(mgpu_engine *e, double *out7) {
          }
          groups.push_back(g);
       }
+      static_assert(sizeof(SkGroup) == sizeof(int4), "SkGroup layout");
+      e->d_groups.reserve(groups.size() + 1);
+      HIPCHK(hipMemcpyAsync(e->d_groups.p, groups.data(), sizeof(SkGroup) * groups.size(),
+                            hipMemcpyHostToDevice, e->stream));
+      const int npart = std::max(1, std::min(16, e->nrows / 8));
+      std::vector<int> rp(npart + 1, e->nrows);
+      rp[0] = 0;
+      int p = 1;
+      for (int q = 0; q < e->nrows && p < npart; q++)
+         if ((long)e->rows[q].kstart * npart >= (long)p * e->nk) rp[p++] = q;
+      for (; p < npart; p++) rp[p] = e->nrows;
+      e->d_rpart.reserve(npart + 1);
+      HIPCHK(hipMemcpyAsync(e->d_rpart.p, rp.data(), sizeof(int) * (npart + 1), hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      e->ew_ngroups = (int)groups.size();
+      e->ew_npart = npart;
+      e->ew_plan_version = e->geom_version;
    }
-   static_assert(sizeof(SkGroup) == sizeof(int4), "SkGroup layout");
-   e->d_groups.reserve(groups.size() + 1);
-   HIPCHK(hipMemcpyAsync(e->d_groups.p, groups.data(), sizeof(SkGroup) * groups.size(),
-                         hipMemcpyHostToDevice, e->stream));
    int nchunks = (e->nsites + CHUNK - 1) / CHUNK;
    e->d_sk.reserve((size_t)nchunks * e->nk * 4 + 4);
    SkArgs s;
@@ -344,7 +345,7 @@ void mgpu::run_ewald_fp64(mgpu_engine *e, double *out7) {
    size_t shz = sizeof(double2) * (e->lmax + 1) * (TS + 1);
    {
       ProfScope ps(e, "ewald_sfac");
-      dim3 g(groups.size(), nchunks);
+      dim3 g(e->ew_ngroups, nchunks);
       if (e->nsitesxf < e->nsites)
          k_sfac<true><<<g, 256, shz, e->stream>>>(s);
       else
@@ -364,20 +365,9 @@ void mgpu::run_ewald_fp64(mgpu_engine *e, double *out7) {
       ProfScope ps(e, "ewald_kspace");
       k_ewald_k<<<nkb, 256, 0, e->stream>>>(ek);
    }
-   e->d_out.reserve(64);
-   reduce_cols(e, e->d_red.p, nkb, 8, e->d_out.p + 8);
-   // row partitions with about equal numbers of k vectors
-   const int npart = std::max(1, std::min(16, e->nrows / 8));
-   std::vector<int> rp(npart + 1, e->nrows);
-   rp[0] = 0;
-   {
-      int p = 1;
-      for (int r = 0; r < e->nrows && p < npart; r++)
-         if ((long)e->rows[r].kstart * npart >= (long)p * e->nk) rp[p++] = r;
-      for (; p < npart; p++) rp[p] = e->nrows;
-   }
-   e->d_rpart.reserve(npart + 1);
-   HIPCHK(hipMemcpyAsync(e->d_rpart.p, rp.data(), sizeof(int) * (npart + 1), hipMemcpyHostToDevice, e->stream));
+   e->d_out.reserve(OUT_SIZE);
+   reduce_cols(e, e->d_red.p, nkb, 8, e->d_out.p + OUT_RECIP);
+   const int npart = e->ew_npart;
    e->d_fpart.reserve((size_t)npart * 3 * nsp);
    EfArgs ef;
    ef.ns = e->nsites; ef.nsp = e->nsp; ef.nsxf = e->nsitesxf; ef.nrows = e->nrows;
@@ -391,5 +381,4 @@ void mgpu::run_ewald_fp64(mgpu_engine *e, double *out7) {
       k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p,
                                                                   e->d_force_site.p);
    }
-   HIPCHK(hipMemcpyAsync(out7, e->d_out.p + 8, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
 }

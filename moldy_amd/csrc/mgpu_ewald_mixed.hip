@@ -552,7 +552,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
-void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
+void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const int ns = e->nsites, nsp = e->nsp, lmax = e->lmax;
    const int NB = (2 * lmax + 1 + 15) / 16;
    if (NB > 6) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (lmax > 47)"};
@@ -610,8 +610,9 @@ void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
    int *dsf = (int *)(B + oSf);
    unsigned *maxbits = (unsigned *)(B + oMx);
    double *bpart = (double *)(B + oBp);
-   // row metadata (host, from the k set)
-   {
+   // row metadata and slices (host, from the k set): uploaded when the k set
+   // or the scratch block changes
+   if (e->mix_plan_version != e->geom_version || e->mix_plan_ptr != (void *)B) {
       std::vector<int2> hk(nrows64, make_int2(-1, 0));
       std::vector<float4> rk(nrows64, make_float4(0, 0, 0, 0));
       for (int r = 0; r < nrows; r++) {
@@ -622,8 +623,11 @@ void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
       HIPCHK(hipMemcpyAsync(rowk, rk.data(), sizeof(float4) * nrows64, hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipMemcpyAsync(dsl, slices.data(), sizeof(int4) * nslices, hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipMemcpyAsync(dsf, slice_fw.data(), sizeof(int) * nslices, hipMemcpyHostToDevice, e->stream));
-      HIPCHK(hipMemsetAsync(maxbits, 0, 4 * sizeof(unsigned), e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      e->mix_plan_version = e->geom_version;
+      e->mix_plan_ptr = (void *)B;
    }
+   zero_async(e, maxbits, 4 * sizeof(unsigned));
    double qmax = 0;
    for (double q : e->site_charge) qmax = std::max(qmax, fabs(q));
    const double sZ = qmax > 0 ? exp2(-ceil(log2(qmax))) : 1.0;
@@ -684,7 +688,6 @@ void mgpu::run_ewald_mixed(mgpu_engine *e, double *out7) {
       run(0, e->nsitesxf, Wt);
       run(e->nsitesxf, ns, Wtn);
    }
-   e->d_out.reserve(64);
-   reduce_cols(e, bpart, nred, 8, e->d_out.p + 24);
-   HIPCHK(hipMemcpyAsync(out7, e->d_out.p + 24, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+   e->d_out.reserve(OUT_SIZE);
+   reduce_cols(e, bpart, nred, 8, e->d_out.p + OUT_RECIP);
 }

@@ -89,14 +89,52 @@ __global__ __launch_bounds__(256) void k_dipole(int ns, int nsp, const double *_
    block_partial<3>(v, part);
 }
 
-__global__ void k_surface(int ns, int nsp, double c0, double c1, double c2, const double *__restrict__ chg,
-                          double *__restrict__ f) {
+// Surface-dipole force, src/accel.c:545-552: f_i -= 4 pi / (3V) q_i M.
+__global__ void k_surface(int ns, int nsp, double c, const double *__restrict__ dip,
+                          const double *__restrict__ chg, double *__restrict__ f) {
    int i = blockIdx.x * 256 + threadIdx.x;
    if (i >= ns) return;
    double q = chg[i];
-   f[i] -= c0 * q;
-   f[nsp + i] -= c1 * q;
-   f[2 * nsp + i] -= c2 * q;
+   f[i] -= c * dip[0] * q;
+   f[nsp + i] -= c * dip[1] * q;
+   f[2 * nsp + i] -= c * dip[2] * q;
+}
+
+struct AsmArgs {
+   double vol, pe_real_shift, dist_vol, distp_vol, self_energy, sheet_vol;
+   int recip, surface;
+};
+
+// Energies, dipole and stress of eval_forces from the reduced sums
+// (src/accel.c:526-606), one thread.
+__global__ void k_assemble(AsmArgs a, double *__restrict__ out) {
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   const double PI_D = 3.14159265358979323846;
+   double pe0 = out[OUT_REAL] + a.pe_real_shift;
+   double pe1 = 0.0, dip[3] = {0, 0, 0}, st_r[6] = {0, 0, 0, 0, 0, 0};
+   if (a.recip) {
+      pe1 += out[OUT_RECIP] - a.self_energy + a.sheet_vol;
+      for (int q = 0; q < 6; q++) st_r[q] = out[OUT_RECIP + 1 + q];
+      st_r[0] += a.sheet_vol;
+      st_r[3] += a.sheet_vol;
+      st_r[5] += a.sheet_vol;
+      for (int c = 0; c < 3; c++) dip[c] = out[OUT_DIP + c];
+      if (a.surface) pe1 += 2.0 * PI_D / (3.0 * a.vol) * (dip[0] * dip[0] + dip[1] * dip[1] + dip[2] * dip[2]);
+   }
+   double st[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+   const int up[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+   for (int q = 0; q < 6; q++) st[3 * up[q][0] + up[q][1]] = out[OUT_REAL + 1 + q] + st_r[q];
+   const double *corr = out + OUT_CORR;
+   for (int i = 0; i < 3; i++) {
+      for (int j = i + 1; j < 3; j++) st[3 * j + i] = st[3 * i + j];
+      for (int j = 0; j < 3; j++) st[3 * i + j] -= corr[3 * i + j] - corr[9 + 3 * i + j];
+   }
+   for (int i = 0; i < 3; i++) st[4 * i] += a.distp_vol;
+   double *r = out + OUT_RES;
+   r[0] = pe0 + a.dist_vol;
+   r[1] = pe1;
+   for (int c = 0; c < 3; c++) r[2 + c] = dip[c];
+   for (int q = 0; q < 9; q++) r[5 + q] = st[q];
 }
 
 struct MfArgs {
@@ -192,20 +230,6 @@ __global__ __launch_bounds__(256) void k_vir_mols(int nmolsxf, double h0, double
    block_partial<9>(v, part);
 }
 
-__global__ void k_sum_cols2(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
-   __shared__ double ws[16];
-   int col = blockIdx.x;
-   double s = 0;
-   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
-   s = wave_sum(s);
-   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      double t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
-      out[col] = t;
-   }
-}
 
 }  // namespace
 
@@ -219,34 +243,26 @@ void mgpu::launch_make_sites(mgpu_engine *e, int mode) {
        e->d_spec_fw.p, e->d_pf.p, e->d_cm.p, e->d_quat.p, e->d_site.p);
 }
 
-// Everything of eval_forces after the two boundary calls.  vir_real: the
-// upper-triangle real-space virial (6), st_recip: the Ewald stress (6).
-void mgpu::run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9], double pe_recip,
-                    const double *vir_real, const double *st_recip) {
+// Everything of eval_forces after the two boundary calls, on the device.
+void mgpu::run_glue(mgpu_engine *e) {
    ProfScope ps(e, "glue");
    const int ns = e->nsites, nsp = e->nsp;
    const int nb = (ns + 255) / 256, nmb = (e->nmols + 255) / 256;
    const double PI_D = 3.14159265358979323846;
    double vol = host_det(e->h);
    e->d_red.reserve((size_t)(nb + nmb) * 9 + 64);
-   e->d_out.reserve(64);
-   double *out = e->d_out.p + 16;
+   e->d_out.reserve(OUT_SIZE);
+   double *out = e->d_out.p;
    // second make_sites: MOLPBC for molecules, SITEPBC for frameworks (accel.c:536-541)
    launch_make_sites(e, 2);
-   double hd[3] = {0, 0, 0};
-   if (e->alpha > 1e-7) {
+   const bool recip = e->alpha > 1e-7;
+   if (recip) {
       k_dipole<<<nb, 256, 0, e->stream>>>(ns, nsp, e->d_site.p, e->d_chg_site.p, e->d_red.p);
-      reduce_cols(e, e->d_red.p, nb, 3, out);
-      HIPCHK(hipMemcpyAsync(hd, out, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-      HIPCHK(hipStreamSynchronize(e->stream));
-      if (e->surface_dipole) {
-         double c = 4.0 * PI_D / (3.0 * vol);
-         k_surface<<<nb, 256, 0, e->stream>>>(ns, nsp, c * hd[0], c * hd[1], c * hd[2], e->d_chg_site.p,
+      reduce_cols(e, e->d_red.p, nb, 3, out + OUT_DIP);
+      if (e->surface_dipole)
+         k_surface<<<nb, 256, 0, e->stream>>>(ns, nsp, 4.0 * PI_D / (3.0 * vol), out + OUT_DIP, e->d_chg_site.p,
                                               e->d_force_site.p);
-         pe_recip += 2.0 * PI_D / (3.0 * vol) * (hd[0] * hd[0] + hd[1] * hd[1] + hd[2] * hd[2]);
-      }
    }
-   dip[0] = hd[0]; dip[1] = hd[1]; dip[2] = hd[2];
    MfArgs mf{e->nmols, nsp};
    k_mol_ft<<<nmb, 256, 0, e->stream>>>(mf, e->d_mol_first.p, e->d_mol_spec.p, e->d_mol_quat.p,
                                         e->d_spec_off.p, e->d_spec_nsites.p, e->d_pf.p, e->d_quat.p,
@@ -260,20 +276,30 @@ void mgpu::run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9
    k_vir_mols<<<nmxb, 256, 0, e->stream>>>(e->nmolsxf, e->h[0], e->h[1], e->h[2], e->h[3], e->h[4],
                                            e->h[5], e->h[6], e->h[7], e->h[8], e->d_cm.p, e->d_mforce.p,
                                            mpart);
-   reduce_cols(e, e->d_red.p, nb, 9, out);
-   reduce_cols(e, mpart, nmxb, 9, out + 9);
-   double corr[18];
-   HIPCHK(hipMemcpyAsync(corr, out, 18 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-   HIPCHK(hipStreamSynchronize(e->stream));
-   // assemble as accel.c:576-606
-   double st[9] = {0};
-   const int up[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
-   for (int q = 0; q < 6; q++) st[3 * up[q][0] + up[q][1]] = vir_real[q] + st_recip[q];
-   for (int i = 0; i < 3; i++) {
-      for (int j = i + 1; j < 3; j++) st[3 * j + i] = st[3 * i + j];
-      for (int j = 0; j < 3; j++) st[3 * i + j] -= corr[3 * i + j] - corr[9 + 3 * i + j];
-   }
-   for (int i = 0; i < 3; i++) st[4 * i] += e->distp / vol;
-   for (int i = 0; i < 9; i++) stress[i] = st[i];
-   pe[1] = pe_recip;
+   reduce_cols(e, e->d_red.p, nb, 9, out + OUT_CORR);
+   reduce_cols(e, mpart, nmxb, 9, out + OUT_CORR + 9);
+   AsmArgs aa;
+   aa.vol = vol;
+   aa.pe_real_shift = (e->precision == MGPU_MIXED && e->intra_exact) ? 0.0 : -e->eintra;
+   aa.dist_vol = e->dist / vol;
+   aa.distp_vol = e->distp / vol;
+   aa.self_energy = e->self_energy;
+   aa.sheet_vol = e->sheet_energy / vol;
+   aa.recip = recip;
+   aa.surface = e->surface_dipole;
+   k_assemble<<<1, 64, 0, e->stream>>>(aa, out);
+}
+
+// Zero fill by a kernel: memset nodes must stay out of captured graphs.
+__global__ void k_zero(uint32_t *__restrict__ p, size_t n4) {
+   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+      p[i] = 0u;
+}
+
+void mgpu::zero_async(mgpu_engine *e, void *p, size_t bytes) {
+   if (bytes % 4) throw StatusError{MGPU_E_ARG, "zero_async: size not a multiple of 4"};
+   size_t n4 = bytes / 4;
+   if (!n4) return;
+   int nb = (int)std::min<size_t>(1024, (n4 + 255) / 256);
+   k_zero<<<nb, 256, 0, e->stream>>>((uint32_t *)p, n4);
 }

@@ -149,6 +149,29 @@ struct mgpu_engine {
    mgpu::DBuf<uint8_t> d_mix;                   // scratch for the MFMA path
    size_t mix_bytes = 0;
 
+   // ---- plan caching: uploads of k-set/slice metadata happen when the
+   // geometry version changes, so a steady-state evaluation is launches only
+   int geom_version = 0;
+   int ew_plan_version = -1, mix_plan_version = -1;
+   int ew_ngroups = 0, ew_npart = 0;
+   void *mix_plan_ptr = nullptr;
+   bool capturing = false;                      // inside hipStreamBeginCapture
+
+   // ---- molecular dynamics (device-resident state of the integrator) ----
+   bool have_dyn = false;
+   mgpu::DBuf<double> d_mom;                    // [nmols][3] scaled momenta
+   mgpu::DBuf<double> d_amom;                   // [nmols_r][4] principal angular momenta
+   mgpu::DBuf<double> d_scal;                   // integrator scalars and KE sums
+   mgpu::DBuf<double> d_trace;                  // per-step record
+   mgpu::DBuf<double> d_spec_dyn;               // per species: mass, I0..2, rI0..2, fw
+   int trace_cap = 0;
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t graph_exec = nullptr;
+   int graph_version = -1, graph_precision = -1;
+   double dyn_step = 0, dyn_ttmass = 0, dyn_pmass = 0, dyn_gkt = 0;
+   int dyn_const_temp = 0, dyn_nosym = 0, dyn_saxis = 0;
+   std::vector<double> spec_mass, spec_inertia;
+
    // ---- diagnostics ----
    mgpu_fc_info last_fc{};
    mgpu_ew_info last_ew{};
@@ -160,18 +183,37 @@ struct mgpu_engine {
 };
 
 namespace mgpu {
+// Slots of the small device result block e->d_out.
+enum {
+   OUT_REAL = 0,    // real space: pe (raw), virial upper triangle (6)
+   OUT_RECIP = 8,   // reciprocal space: pe, stress upper triangle (6)
+   OUT_DIP = 16,    // total dipole (3)
+   OUT_CORR = 24,   // site- and molecule-virial corrections (9 + 9)
+   OUT_RES = 48,    // eval_forces results: pe[2], dip[3], stress[9]
+   OUT_TMP = 64,    // column sums copied to the host at once (16)
+   OUT_SIZE = 80
+};
+// Flags block e->d_iflags: [0] error bits, [1] too-close count, [2..33] pairs,
+// [40] first molecule whose quaternion norm failed.
+enum { FLAG_BIN_RANGE = 1, FLAG_BIN_ROUND = 2, FLAG_QNORM = 4 };
+
 // launch helpers implemented per translation unit
 void launch_make_sites(mgpu_engine *e, int molflag_mode);
 void run_binning(mgpu_engine *e, bool site_only_cellorder);
-void run_realspace(mgpu_engine *e, double *pe_vir_out /*7*/);
+void run_realspace(mgpu_engine *e);          // -> d_out[OUT_REAL..]
 void run_pairlist_count(mgpu_engine *e, int strict, double cutoff, const int *offs, int noff,
                         long *total, long *list_start_host);
 void run_pairlist_fill(mgpu_engine *e, const int *offs, int noff, long total, int *nab,
                        int *pbctrans, int *nfnab);
-void run_ewald_fp64(mgpu_engine *e, double *pe_stress_out /*7*/);
-void run_ewald_mixed(mgpu_engine *e, double *pe_stress_out /*7*/);
-void run_glue(mgpu_engine *e, double pe[2], double dip[3], double stress[9], double pe_recip,
-              const double *vir_real, const double *st_recip);
+void run_ewald_fp64(mgpu_engine *e);        // -> d_out[OUT_RECIP..]
+void run_ewald_mixed(mgpu_engine *e);       // -> d_out[OUT_RECIP..]
+// The device part of eval_forces after the two boundary calls: second
+// make_sites, dipole and surface term, molecular forces and torques, virial
+// correction and the assembly of pe, dipole and stress into d_out[OUT_RES..].
+void run_glue(mgpu_engine *e);
+void eval_forces_device(mgpu_engine *e, bool clear_flags);
+// Fill with zeros by a kernel (safe inside a captured graph).
+void zero_async(mgpu_engine *e, void *p, size_t bytes);
 void device_scan_int(mgpu_engine *e, const int *in, int *out, int n);  // exclusive, out[n]=total
 void sum_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *host_out);
 void reduce_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *out_dev);

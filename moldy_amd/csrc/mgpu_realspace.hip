@@ -312,7 +312,7 @@ void launch_pt(int pt, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
 
 }  // namespace
 
-void mgpu::run_realspace(mgpu_engine *e, double *out7) {
+void mgpu::run_realspace(mgpu_engine *e) {
    int nc = e->ncells;
    e->d_cell_part.reserve((size_t)nc * 8 + 8);
    RSArgs a;
@@ -357,7 +357,6 @@ void mgpu::run_realspace(mgpu_engine *e, double *out7) {
          else launch_pt<float, false>(e->ptype, g, b, e->stream, a);
       }
    }
-   e->d_out.reserve(64);
-   reduce_cols(e, e->d_cell_part.p, nc, 8, e->d_out.p);
-   HIPCHK(hipMemcpyAsync(out7, e->d_out.p, 7 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+   e->d_out.reserve(OUT_SIZE);
+   reduce_cols(e, e->d_cell_part.p, nc, 8, e->d_out.p + OUT_REAL);
 }
