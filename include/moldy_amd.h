@@ -103,6 +103,39 @@ int mgpu_download_forces(mgpu_engine *eng, double *force, double *torque,
 /* Host-side copies of the last call's diagnostics. */
 int mgpu_last_info(const mgpu_engine *eng, mgpu_fc_info *fc, mgpu_ew_info *ew);
 
+/* ---- molecular dynamics on the device (do_step, src/accel.c:626-827) ----
+ * The leapfrog splitting of src/leapfrog.c in FP64: centre-of-mass update
+ * with the metric (h^T h)^-1, momentum half kicks, the symplectic rotation
+ * of leapf_quat_b (or leapf_quat_a when nosymmetric_rot) and, when
+ * const_temp, the Nose-Poincare thermostat.  The cell is fixed (no barostat).
+ * Forces come from mgpu_eval_forces' device path at the engine's precision. */
+typedef struct {
+   double step;                  /* control.step, program time units          */
+   double ttmass;                /* control.ttmass                            */
+   double pmass;                 /* control.pmass (enters H_0 via ke_cell)    */
+   double gkt;                   /* sys->d_of_f * kB * control.temp           */
+   int const_temp;               /* 0, or nonzero = Nose-Poincare thermostat  */
+   int nosymmetric_rot;          /* control.nosymmetric_rot                   */
+   const double *spec_mass;      /* [nspecies] molecular masses               */
+   const double *spec_inertia;   /* [nspecies][3] principal moments           */
+} mgpu_dynamics;
+
+int mgpu_set_dynamics(mgpu_engine *eng, const mgpu_dynamics *dyn);
+/* Scaled momenta mom[nmols][3] (mom = h^T p, as spec_mt.mom), principal
+ * angular momenta amom[nmols_r][4] and thermo = {s, p_s, H_0}.  Positions and
+ * h come from mgpu_upload_state. */
+int mgpu_upload_momenta(mgpu_engine *eng, const double *mom, const double *amom,
+                        const double thermo[3]);
+/* Runs nsteps do_steps on the device.  init_H_0: evaluate H_0 on the first
+ * step (accel.c:718-724).  use_graph: replay steps 2..nsteps from a captured
+ * hipGraph.  trace (optional, [nsteps][8]) receives per step: translational
+ * KE, rotational KE, pe[0], pe[1] at the H_0 point, s, p_s, H_0, 0.  pe and
+ * stress (optional) are those of the last step's force evaluation. */
+int mgpu_md_steps(mgpu_engine *eng, int nsteps, int init_H_0, int use_graph, double *trace,
+                  double pe[2], double stress[9]);
+int mgpu_download_dynamics(mgpu_engine *eng, double *c_of_m_frac, double *quat, double *mom,
+                           double *amom, double thermo[3]);
+
 /* ---- link cells and pair lists (bit-exact with src/force.c:426-569) ----
  * mgpu_cells: per entry (a site, or a molecule when molpbc) its cell, and the
  * cell lists in the reference's linked-list order as CSR.

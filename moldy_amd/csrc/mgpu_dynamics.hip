@@ -1,0 +1,528 @@
+// Molecular-dynamics step on the device: do_step of src/accel.c:626-827 with
+// the leapfrog splitting of src/leapfrog.c (centre-of-mass update with the
+// metric G^-1, momentum kicks, the symplectic quaternion rotations of
+// leapf_quat_a/leapf_quat_b) and the Nose-Poincare thermostat, in FP64.
+// The cell matrix is constant (no barostat), so every launch of a step has
+// fixed arguments and the whole step can be captured in a hipGraph.
+//
+// Device layout: one thread per molecule; momenta mom[nmols][3] (scaled,
+// mom = h^T p), principal angular momenta amom[nmols_r][4]; the integrator
+// scalars (s, p_s, H_0, record counter) and per-species kinetic sums live
+// in e->d_scal so that no value goes back to the host inside a step.
+#include "mgpu_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+using namespace mgpu;
+
+namespace {
+
+constexpr double INERTIA_MIN = 1.0e-14;   // src/leapfrog.c:75
+constexpr double QTOL = 1.0e-4;           // src/leapfrog.c:72
+constexpr double DMIN = 2.2250738585072014e-308;
+
+// d_scal slots
+enum { S_TS = 0, S_TSMOM = 1, S_H0 = 2, S_ROW = 3, S_KE = 8 };
+constexpr int MAX_SPEC = 14;              // S_KE + 4 * MAX_SPEC <= 64
+constexpr int SCAL_SIZE = 64;
+constexpr int TRACE_W = 8;                // ke_t, ke_r, pe0, pe1, ts, tsmom, H_0, 0
+
+struct SpecDyn {
+   double mass, I[3], rI[3], fw;
+};
+
+__device__ __forceinline__ void qmul(const double *p, const double *q, double *r) {
+   double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+   double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+   r[0] = p0 * q0 - p1 * q1 - p2 * q2 - p3 * q3;
+   r[1] = p1 * q0 + p0 * q1 - p3 * q2 + p2 * q3;
+   r[2] = p2 * q0 + p3 * q1 + p0 * q2 - p1 * q3;
+   r[3] = p3 * q0 - p2 * q1 + p1 * q2 + p0 * q3;
+}
+
+// rot_substep, src/leapfrog.c:238-244
+__device__ __forceinline__ void rot_substep(const double *rot, double *L, double *q) {
+   double rc[4] = {rot[0], -rot[1], -rot[2], -rot[3]};
+   qmul(rc, L, L);
+   qmul(L, rot, L);
+   qmul(q, rot, q);
+}
+
+// make_rot, src/leapfrog.c:193-208: rotation about a principal axis
+__device__ __forceinline__ void rot_axis(double step, int axis, const double *L, double rinertia, double *rot) {
+   double s, c;
+   sincos(0.5 * step * rinertia * L[axis + 1], &s, &c);
+   rot[0] = c;
+   rot[1] = rot[2] = rot[3] = 0.0;
+   rot[axis + 1] = s;
+}
+
+// make_rot_amom, src/leapfrog.c:213-231: rotation about L
+__device__ __forceinline__ void rot_amom(double step, const double *L, double rinertia, double *rot) {
+   double samom = sqrt(L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
+   double ramom = 1.0 / (samom + 8 * DMIN);
+   double s, c;
+   sincos(0.5 * step * rinertia * samom, &s, &c);
+   rot[0] = c;
+   rot[1] = s * ramom * L[1];
+   rot[2] = s * ramom * L[2];
+   rot[3] = s * ramom * L[3];
+}
+
+struct CoordArgs {
+   int nmols;
+   double step;          // the half step of leapf_all_coords
+   double Gi[9];         // (h^T h)^-1
+   int do_com;           // centre-of-mass update in this launch
+   int ph_lo, ph_hi;     // rotation sub-steps [ph_lo, ph_hi) of the splitting
+   int variant_a;        // leapf_quat_a (else leapf_quat_b)
+   int saxis;            // near-symmetry axis of leapf_quat_b
+   int normalise;        // normalise after the last sub-step
+};
+
+// leapf_all_coords (src/accel.c:360-372): leapf_com (leapfrog.c:133-146) and
+// the sub-steps [ph_lo, ph_hi) of leapf_quat_b (leapfrog.c:255-338: 0 half
+// delta about o1, 1 precession about the symmetry axis and spin about L, 2
+// half delta about o1) or leapf_quat_a (leapfrog.c:345-392: axes 0,1,2,1,0
+// with weights 1/2,1/2,1,1/2,1/2), then normalise (leapfrog.c:96-113).
+__global__ void k_coords(CoordArgs a, const int *__restrict__ mol_spec, const int *__restrict__ mol_quat,
+                         const SpecDyn *__restrict__ sd, const double *__restrict__ scal,
+                         const double *__restrict__ mom, double *__restrict__ cm, double *__restrict__ quat,
+                         double *__restrict__ amom, int *__restrict__ flags) {
+   int m = blockIdx.x * blockDim.x + threadIdx.x;
+   if (m >= a.nmols) return;
+   const int sp = mol_spec[m];
+   const SpecDyn S = sd[sp];
+   const double ts = scal[S_TS];
+   if (a.do_com) {
+      const double c = a.step / (S.mass * ts);
+      double A[9];
+#pragma unroll
+      for (int i = 0; i < 9; i++) A[i] = c * a.Gi[i];
+      double *r = cm + 3 * m;
+      const double *p = mom + 3 * m;
+      double y0 = r[0] + A[0] * p[0] + A[1] * p[1] + A[2] * p[2];
+      double y1 = r[1] + A[3] * p[0] + A[4] * p[1] + A[5] * p[2];
+      double y2 = r[2] + A[6] * p[0] + A[7] * p[1] + A[8] * p[2];
+      // escape, leapfrog.c:117-128
+      r[0] = y0 - floor(y0 + 0.5);
+      r[1] = y1 - floor(y1 + 0.5);
+      r[2] = y2 - floor(y2 + 0.5);
+   }
+   const int iq = mol_quat[m];
+   if (iq < 0 || a.ph_hi <= a.ph_lo) return;
+   double q[4], L[4], rot[4];
+#pragma unroll
+   for (int k = 0; k < 4; k++) { q[k] = quat[4 * iq + k]; L[k] = amom[4 * iq + k]; }
+   const double stepdts = a.step / ts;
+   for (int ph = a.ph_lo; ph < a.ph_hi; ph++) {
+      if (a.variant_a) {
+         const int ax = ph == 0 || ph == 4 ? 0 : (ph == 2 ? 2 : 1);
+         const double w = ph == 2 ? 1.0 : 0.5;
+         rot_axis(w * stepdts, ax, L, S.rI[ax], rot);
+         rot_substep(rot, L, q);
+      } else {
+         const int s = a.saxis, o1 = (s + 1) % 3, o2 = (s + 2) % 3;
+         if (ph == 1) {
+            rot_axis(stepdts, s, L, S.rI[s] - S.rI[o2], rot);
+            rot_substep(rot, L, q);
+            rot_amom(stepdts, L, S.rI[o2], rot);
+            qmul(q, rot, q);
+         } else {
+            rot_axis(0.5 * stepdts, o1, L, S.rI[o1] - S.rI[o2], rot);
+            rot_substep(rot, L, q);
+         }
+      }
+   }
+   if (a.normalise) {
+      double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+      if (fabs(n - 1.0) > QTOL) {
+         atomicOr(flags, FLAG_QNORM);
+         atomicCAS(flags + 40, 0, m + 1);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) q[k] /= n;
+   }
+#pragma unroll
+   for (int k = 0; k < 4; k++) { quat[4 * iq + k] = q[k]; amom[4 * iq + k] = L[k]; }
+}
+
+struct MomArgs {
+   int nmols;
+   double step;          // the half step of leapf_all_momenta
+   double hT[9];         // h^T
+   int zero_fw;          // zero framework momenta afterwards (accel.c:752-757)
+};
+
+// leapf_all_momenta, src/accel.c:376-388 (leapf_mom, leapf_amom)
+__global__ void k_momenta(MomArgs a, const int *__restrict__ mol_spec, const int *__restrict__ mol_quat,
+                          const SpecDyn *__restrict__ sd, const double *__restrict__ scal,
+                          const double *__restrict__ force, const double *__restrict__ torque,
+                          double *__restrict__ mom, double *__restrict__ amom) {
+   int m = blockIdx.x * blockDim.x + threadIdx.x;
+   if (m >= a.nmols) return;
+   const double c = a.step * scal[S_TS];
+   double A[9];
+#pragma unroll
+   for (int i = 0; i < 9; i++) A[i] = c * a.hT[i];
+   double *p = mom + 3 * m;
+   const double *f = force + 3 * m;
+   double y0 = p[0] + A[0] * f[0] + A[1] * f[1] + A[2] * f[2];
+   double y1 = p[1] + A[3] * f[0] + A[4] * f[1] + A[5] * f[2];
+   double y2 = p[2] + A[6] * f[0] + A[7] * f[1] + A[8] * f[2];
+   if (a.zero_fw && sd[mol_spec[m]].fw != 0.0) y0 = y1 = y2 = 0.0;
+   p[0] = y0; p[1] = y1; p[2] = y2;
+   const int iq = mol_quat[m];
+   if (iq >= 0) {
+      amom[4 * iq + 1] += c * torque[3 * iq];
+      amom[4 * iq + 2] += c * torque[3 * iq + 1];
+      amom[4 * iq + 3] += c * torque[3 * iq + 2];
+   }
+}
+
+struct KeArgs {
+   int nmols, nspec;
+   double hiT[9];        // (h^-1)^T: real momenta
+};
+
+// Per-species sums of |p_real|^2 and amom_k^2 (k = 1..3): columns 4s+0..3
+// of a [nblocks][4 nspec] partial array (trans_ke and rot_ke numerators,
+// src/algorith.c:221-259).  Deterministic: wave sums, fixed block order.
+__global__ __launch_bounds__(256) void k_ke_part(KeArgs a, const int *__restrict__ mol_spec,
+                                                 const int *__restrict__ mol_quat, const double *__restrict__ mom,
+                                                 const double *__restrict__ amom, double *__restrict__ part) {
+   __shared__ double ws[4][4 * MAX_SPEC];
+   int m = blockIdx.x * 256 + threadIdx.x;
+   double v[4] = {0, 0, 0, 0};
+   int sp = -1;
+   if (m < a.nmols) {
+      sp = mol_spec[m];
+      const double *q = mom + 3 * m;
+      double p0 = a.hiT[0] * q[0] + a.hiT[1] * q[1] + a.hiT[2] * q[2];
+      double p1 = a.hiT[3] * q[0] + a.hiT[4] * q[1] + a.hiT[5] * q[2];
+      double p2 = a.hiT[6] * q[0] + a.hiT[7] * q[1] + a.hiT[8] * q[2];
+      v[0] = p0 * p0 + p1 * p1 + p2 * p2;
+      const int iq = mol_quat[m];
+      if (iq >= 0) {
+         v[1] = amom[4 * iq + 1] * amom[4 * iq + 1];
+         v[2] = amom[4 * iq + 2] * amom[4 * iq + 2];
+         v[3] = amom[4 * iq + 3] * amom[4 * iq + 3];
+      }
+   }
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   for (int s = 0; s < a.nspec; s++)
+      for (int k = 0; k < 4; k++) {
+         double t = wave_sum(sp == s ? v[k] : 0.0);
+         if (lane == 0) ws[w][4 * s + k] = t;
+      }
+   __syncthreads();
+   const int nc = 4 * a.nspec;
+   if (threadIdx.x < nc)
+      part[(size_t)blockIdx.x * nc + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+// Column sums of the partials into scal[S_KE ..], one thread per column.
+__global__ void k_ke_sum(int nblocks, int nc, const double *__restrict__ part, double *__restrict__ scal) {
+   int c = threadIdx.x;
+   if (c >= nc) return;
+   double t = 0.0;
+   for (int b = 0; b < nblocks; b++) t += part[(size_t)b * nc + c];
+   scal[S_KE + c] = t;
+}
+
+enum {
+   OP_THERM_IN = 0,     // leapf_nose_therm then the log term (accel.c:680-682)
+   OP_THERM_OUT,        // the log term then leapf_nose_therm (accel.c:788-789)
+   OP_TSMOM_TRANS,      // tsmom += 1/2 step KE_trans (accel.c:698-699, 765-766)
+   OP_TSMOM_ROT,        // tsmom += 1/2 h rot_ke (leapfrog.c:298-299, 317-318)
+   OP_TSMOM_AXES,       // tsmom += w * stepdts2 * rI_k * sum amom_k^2 (leapfrog.c:364-381)
+   OP_RECORD,           // KE at the H_0 point, H_0, trace row (accel.c:714-724)
+   OP_TSMOM_PE,         // tsmom -= step (pe - H_0) (accel.c:759-760)
+   OP_RESET_ROW
+};
+
+struct ScalArgs {
+   int op, nspec, init_h0, trace_cap;
+   double step, ttmass, pmass, gkt;
+   double w[3];                 // per-axis weights (OP_TSMOM_AXES), order of addition
+   int axes[3], naxes;
+};
+
+__device__ double ke_trans(const ScalArgs &a, const SpecDyn *sd, const double *scal) {
+   double ts = scal[S_TS], ke = 0.0;
+   for (int s = 0; s < a.nspec; s++) ke += scal[S_KE + 4 * s] / (2.0 * sd[s].mass * ts * ts);
+   return ke;
+}
+
+__device__ double ke_rot(const ScalArgs &a, const SpecDyn *sd, const double *scal, int s) {
+   double ts = scal[S_TS], ke = 0.0;
+   for (int i = 0; i < 3; i++)
+      if (sd[s].I[i] > INERTIA_MIN) ke += scal[S_KE + 4 * s + 1 + i] / sd[s].I[i];
+   return 0.5 * ke / (ts * ts);
+}
+
+__global__ void k_scalar(ScalArgs a, const SpecDyn *__restrict__ sd, const int *__restrict__ spec_rdof,
+                         const double *__restrict__ res, double *__restrict__ scal, double *__restrict__ trace) {
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   double &ts = scal[S_TS], &tsmom = scal[S_TSMOM];
+   switch (a.op) {
+      case OP_THERM_IN:
+      case OP_THERM_OUT: {
+         const double h = 0.5 * a.step;
+         if (a.op == OP_THERM_OUT) tsmom -= h * a.gkt * (1.0 + log(ts));
+         const double scale = 1.0 + tsmom * h / (2.0 * a.ttmass);   // leapfrog.c:468-474
+         tsmom /= scale;
+         ts *= scale * scale;
+         if (a.op == OP_THERM_IN) tsmom -= h * a.gkt * (1.0 + log(ts));
+         break;
+      }
+      case OP_TSMOM_TRANS: tsmom += 0.5 * a.step * ke_trans(a, sd, scal); break;
+      case OP_TSMOM_ROT:
+         for (int s = 0; s < a.nspec; s++)
+            if (spec_rdof[s] > 0) tsmom += 0.5 * a.step * ke_rot(a, sd, scal, s);
+         break;
+      case OP_TSMOM_AXES: {
+         const double stepdts2 = a.step / (ts * ts);
+         for (int s = 0; s < a.nspec; s++)
+            if (spec_rdof[s] > 0)
+               for (int k = 0; k < a.naxes; k++)
+                  tsmom += a.w[k] * stepdts2 * sd[s].rI[a.axes[k]] * scal[S_KE + 4 * s + 1 + a.axes[k]];
+         break;
+      }
+      case OP_RECORD: {
+         double kt = ke_trans(a, sd, scal), kr = 0.0;
+         for (int s = 0; s < a.nspec; s++)
+            if (spec_rdof[s] > 0) kr += ke_rot(a, sd, scal, s);
+         const double pe0 = res[0], pe1 = res[1];
+         if (a.init_h0)
+            scal[S_H0] = (kt + kr) + (pe0 + pe1) + tsmom * tsmom / (2.0 * a.ttmass) + a.gkt * log(ts) +
+                         0.5 * 0.0 / a.pmass;
+         int row = (int)scal[S_ROW];
+         if (row < a.trace_cap) {
+            double *t = trace + (size_t)row * TRACE_W;
+            t[0] = kt; t[1] = kr; t[2] = pe0; t[3] = pe1; t[4] = ts; t[5] = tsmom; t[6] = scal[S_H0];
+            t[7] = 0.0;
+         }
+         scal[S_ROW] = row + 1;
+         break;
+      }
+      case OP_TSMOM_PE: tsmom -= a.step * (res[0] + res[1] - scal[S_H0]); break;
+      case OP_RESET_ROW: scal[S_ROW] = 0.0; break;
+   }
+}
+
+void matmul3(const double *a, const double *b, double *c) {
+   for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+         double t = 0;
+         for (int k = 0; k < 3; k++) t += a[3 * i + k] * b[3 * k + j];
+         c[3 * i + j] = t;
+      }
+}
+
+ScalArgs scal_args(mgpu_engine *e, int op) {
+   ScalArgs a{};
+   a.op = op;
+   a.nspec = e->nspecies;
+   a.trace_cap = e->trace_cap;
+   a.step = e->dyn_step;
+   a.ttmass = e->dyn_ttmass;
+   a.pmass = e->dyn_pmass;
+   a.gkt = e->dyn_gkt;
+   return a;
+}
+
+void scalar_op(mgpu_engine *e, const ScalArgs &a) {
+   k_scalar<<<1, 64, 0, e->stream>>>(a, (const SpecDyn *)e->d_spec_dyn.p, e->d_spec_rdof.p,
+                                     e->d_out.p + OUT_RES, e->d_scal.p, e->d_trace.p);
+}
+
+void ke_sums(mgpu_engine *e) {
+   KeArgs k;
+   k.nmols = e->nmols;
+   k.nspec = e->nspecies;
+   double hiT[9];
+   for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) hiT[3 * i + j] = e->hinv[3 * j + i];
+   for (int i = 0; i < 9; i++) k.hiT[i] = hiT[i];
+   const int nb = (e->nmols + 255) / 256, nc = 4 * e->nspecies;
+   k_ke_part<<<nb, 256, 0, e->stream>>>(k, e->d_mol_spec.p, e->d_mol_quat.p, e->d_mom.p, e->d_amom.p,
+                                        e->d_red.p);
+   k_ke_sum<<<1, 64, 0, e->stream>>>(nb, nc, e->d_red.p, e->d_scal.p);
+}
+
+// One leapf_all_coords half step, with the thermostat's rotational terms
+// between the rotation sub-steps when the thermostat is on.
+void coords_half(mgpu_engine *e) {
+   CoordArgs c{};
+   c.nmols = e->nmols;
+   c.step = 0.5 * e->dyn_step;
+   double ht[9], G[9];
+   for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) ht[3 * i + j] = e->h[3 * j + i];
+   matmul3(ht, e->h, G);
+   host_invert(G, c.Gi);
+   c.variant_a = e->dyn_nosym;
+   c.saxis = e->dyn_saxis;
+   const int nb = (e->nmols + 255) / 256;
+   const int nph = e->dyn_nosym ? 5 : 3;
+   auto launch = [&](int com, int lo, int hi) {
+      c.do_com = com;
+      c.ph_lo = lo;
+      c.ph_hi = hi;
+      c.normalise = hi == nph;
+      k_coords<<<nb, 256, 0, e->stream>>>(c, e->d_mol_spec.p, e->d_mol_quat.p, (const SpecDyn *)e->d_spec_dyn.p,
+                                          e->d_scal.p, e->d_mom.p, e->d_cm.p, e->d_quat.p, e->d_amom.p,
+                                          e->d_iflags.p);
+   };
+   if (!e->dyn_const_temp || e->nmols_r == 0) {
+      launch(1, 0, e->nmols_r ? nph : 0);
+      return;
+   }
+   if (!e->dyn_nosym) {
+      launch(1, 0, 1);
+      ke_sums(e);
+      ScalArgs a = scal_args(e, OP_TSMOM_ROT);
+      a.step = c.step;
+      scalar_op(e, a);
+      launch(0, 1, 2);
+      ke_sums(e);
+      scalar_op(e, a);
+      launch(0, 2, 3);
+   } else {
+      // leapf_quat_a: sums after sub-steps 0, 1 and 3
+      launch(1, 0, 1);
+      ke_sums(e);
+      ScalArgs a = scal_args(e, OP_TSMOM_AXES);
+      a.step = c.step;
+      a.naxes = 2; a.axes[0] = 0; a.axes[1] = 1; a.w[0] = 0.25; a.w[1] = 0.25;
+      scalar_op(e, a);
+      launch(0, 1, 2);
+      ke_sums(e);
+      a.naxes = 1; a.axes[0] = 2; a.w[0] = 0.5;
+      scalar_op(e, a);
+      launch(0, 2, 4);
+      ke_sums(e);
+      a.naxes = 2; a.axes[0] = 1; a.axes[1] = 0; a.w[0] = 0.25; a.w[1] = 0.25;
+      scalar_op(e, a);
+      launch(0, 4, 5);
+   }
+}
+
+void momenta_half(mgpu_engine *e, bool zero_fw) {
+   MomArgs m{};
+   m.nmols = e->nmols;
+   m.step = 0.5 * e->dyn_step;
+   for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) m.hT[3 * i + j] = e->h[3 * j + i];
+   m.zero_fw = zero_fw;
+   k_momenta<<<(e->nmols + 255) / 256, 256, 0, e->stream>>>(m, e->d_mol_spec.p, e->d_mol_quat.p,
+                                                            (const SpecDyn *)e->d_spec_dyn.p, e->d_scal.p,
+                                                            e->d_mforce.p, e->d_mtorque.p, e->d_mom.p,
+                                                            e->d_amom.p);
+}
+
+}  // namespace
+
+void mgpu::md_setup(mgpu_engine *e, const mgpu_dynamics *d) {
+   if (e->nspecies > MAX_SPEC) throw StatusError{MGPU_E_CAPACITY, "more than 14 species in the MD step"};
+   if (!(d->step > 0)) throw StatusError{MGPU_E_ARG, "step must be positive"};
+   if (d->const_temp && !(d->ttmass > 0)) throw StatusError{MGPU_E_ARG, "ttmass must be positive"};
+   std::vector<SpecDyn> sd(e->nspecies);
+   e->spec_mass.assign(d->spec_mass, d->spec_mass + e->nspecies);
+   e->spec_inertia.assign(d->spec_inertia, d->spec_inertia + 3 * e->nspecies);
+   int saxis = 0;
+   bool first = true;
+   for (int s = 0; s < e->nspecies; s++) {
+      if (!(d->spec_mass[s] > 0)) throw StatusError{MGPU_E_ARG, "species mass must be positive"};
+      SpecDyn &S = sd[s];
+      S.mass = d->spec_mass[s];
+      const double *I = d->spec_inertia + 3 * s;
+      for (int i = 0; i < 3; i++) {
+         S.I[i] = I[i];
+         // leapfrog.c:263-269
+         S.rI[i] = (e->spec_rdof[s] > 0 && !(I[i] / (I[(i + 1) % 3] + I[(i + 2) % 3]) < INERTIA_MIN)) ? 1.0 / I[i]
+                                                                                                     : 0.0;
+      }
+      S.fw = e->spec_fw[s] ? 1.0 : 0.0;
+      if (e->spec_rdof[s] > 0 && first) {
+         // near-symmetry axis, leapfrog.c:275-290; the reference keeps the
+         // first species' choice for all later ones (a static)
+         double idmin = 1.0e37;
+         for (int i = 0; i < 3; i++) {
+            double idiff = fabs(S.rI[(i + 1) % 3] - S.rI[(i + 2) % 3]);
+            if (idiff < idmin) { idmin = idiff; saxis = i; }
+         }
+         first = false;
+      }
+   }
+   e->dyn_step = d->step;
+   e->dyn_ttmass = d->ttmass;
+   e->dyn_pmass = d->pmass;
+   e->dyn_gkt = d->gkt;
+   e->dyn_const_temp = d->const_temp != 0;
+   e->dyn_nosym = d->nosymmetric_rot != 0;
+   e->dyn_saxis = saxis;
+   e->d_spec_dyn.reserve(sizeof(SpecDyn) / sizeof(double) * e->nspecies);
+   HIPCHK(hipMemcpy(e->d_spec_dyn.p, sd.data(), sizeof(SpecDyn) * sd.size(), hipMemcpyHostToDevice));
+   e->d_mom.reserve(3 * (size_t)e->nmols + 3);
+   e->d_amom.reserve(4 * (size_t)std::max(e->nmols_r, 1));
+   e->d_scal.reserve(SCAL_SIZE);
+   const int nb = (e->nmols + 255) / 256;
+   e->d_red.reserve((size_t)nb * 4 * MAX_SPEC + 64);
+   e->have_dyn = true;
+}
+
+void mgpu::md_set_scalars(mgpu_engine *e, const double thermo[3]) {
+   double v[SCAL_SIZE] = {0};
+   v[S_TS] = thermo[0];
+   v[S_TSMOM] = thermo[1];
+   v[S_H0] = thermo[2];
+   HIPCHK(hipMemcpyAsync(e->d_scal.p, v, sizeof v, hipMemcpyHostToDevice, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+}
+
+void mgpu::md_get_scalars(mgpu_engine *e, double thermo[3]) {
+   double v[4];
+   HIPCHK(hipMemcpyAsync(v, e->d_scal.p, sizeof v, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   thermo[0] = v[S_TS];
+   thermo[1] = v[S_TSMOM];
+   thermo[2] = v[S_H0];
+}
+
+void mgpu::md_reset_record(mgpu_engine *e, int cap) {
+   e->d_trace.reserve((size_t)std::max(cap, 1) * TRACE_W);
+   e->trace_cap = cap;
+   scalar_op(e, scal_args(e, OP_RESET_ROW));
+}
+
+// do_step, src/accel.c:626-827, at constant h, enqueued on e->stream with
+// no host synchronisation.
+void mgpu::md_step_launch(mgpu_engine *e, bool init_h0) {
+   const bool therm = e->dyn_const_temp;
+   if (therm) scalar_op(e, scal_args(e, OP_THERM_IN));
+   if (therm) {
+      ke_sums(e);
+      scalar_op(e, scal_args(e, OP_TSMOM_TRANS));
+   }
+   coords_half(e);
+   eval_forces_device(e, false);
+   momenta_half(e, false);
+   ke_sums(e);
+   ScalArgs r = scal_args(e, OP_RECORD);
+   r.init_h0 = init_h0;
+   scalar_op(e, r);
+   momenta_half(e, true);
+   if (therm) scalar_op(e, scal_args(e, OP_TSMOM_PE));
+   if (therm) {
+      ke_sums(e);
+      scalar_op(e, scal_args(e, OP_TSMOM_TRANS));
+   }
+   coords_half(e);
+   if (therm) scalar_op(e, scal_args(e, OP_THERM_OUT));
+}
+
+int mgpu::md_trace_width() { return TRACE_W; }

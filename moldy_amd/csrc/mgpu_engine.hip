@@ -441,7 +441,7 @@ void handle_flags(mgpu_engine *e, const int *fl) {
    if (fl[0] & FLAG_BIN_ROUND) throw StatusError{MGPU_E_BIN_ROUND, "Rounding problem in BIN (fill_cells)"};
    if (fl[0] & FLAG_QNORM)
       throw StatusError{MGPU_E_QNORM, "quaternion normalisation failed (leapfrog.c normalise) at molecule " +
-                                          std::to_string(fl[40])};
+                                          std::to_string(fl[40] - 1)};
    mgpu_fc_info fi{};
    fi.nx = e->nx; fi.ny = e->ny; fi.nz = e->nz; fi.n_nabors = e->n_half;
    fi.n_too_close = fl[1];
@@ -461,7 +461,7 @@ void sync_flags(mgpu_engine *e) {
 // forces accumulate into e->d_force_site, sums into d_out[OUT_REAL..].
 void real_pass_device(mgpu_engine *e, bool clear_flags) {
    if (e->max_id > MGPU_MAX_ID) throw StatusError{MGPU_E_CAPACITY, "max_id > 16"};
-   if (clear_flags) zero_async(e, e->d_iflags.p, sizeof(int) * 40);
+   if (clear_flags) zero_async(e, e->d_iflags.p, sizeof(int) * 48);
    run_binning(e, false);
    run_realspace(e);
 }
@@ -544,8 +544,10 @@ __global__ void k_pre_virial(int ns, int nsp, const double *__restrict__ s, cons
           ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
 }
 
+unsigned long mgpu::g_alloc_epoch = 0;
+
 mgpu::ProfScope::ProfScope(mgpu_engine *e_, const char *n) : e(e_), name(n) {
-   if (!e->prof_on) return;
+   if (!e->prof_on || e->capturing) return;
    if (e->ev_used * 2 + 2 > (int)e->ev_pool.size()) {
       for (int k = 0; k < 64; k++) {
          hipEvent_t ev;
@@ -614,6 +616,9 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
    fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_mom); fr(e->d_amom); fr(e->d_scal); fr(e->d_trace); fr(e->d_spec_dyn);
+   if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+   if (e->graph) (void)hipGraphDestroy(e->graph);
    for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
    if (e->stream) (void)hipStreamDestroy(e->stream);
    delete e;
@@ -828,6 +833,107 @@ int mgpu_download_forces(mgpu_engine *e, double *force, double *torque, double *
    });
 }
 
+int mgpu_set_dynamics(mgpu_engine *e, const mgpu_dynamics *d) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      if (!d || !d->spec_mass || !d->spec_inertia) return fail(e, MGPU_E_ARG, "bad dynamics description");
+      HIPCHK(hipSetDevice(e->device));
+      md_setup(e, d);
+      return 0;
+   });
+}
+
+int mgpu_upload_momenta(mgpu_engine *e, const double *mom, const double *amom, const double thermo[3]) {
+   return guarded(e, [&] {
+      if (!e->have_dyn) return fail(e, MGPU_E_STATE, "mgpu_set_dynamics not called");
+      if (!mom || (e->nmols_r && !amom) || !thermo) return fail(e, MGPU_E_ARG, "bad buffers");
+      if (!(thermo[0] > 0)) return fail(e, MGPU_E_ARG, "thermostat s must be positive");
+      HIPCHK(hipSetDevice(e->device));
+      HIPCHK(hipMemcpyAsync(e->d_mom.p, mom, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
+      if (e->nmols_r)
+         HIPCHK(hipMemcpyAsync(e->d_amom.p, amom, sizeof(double) * 4 * e->nmols_r, hipMemcpyHostToDevice,
+                               e->stream));
+      md_set_scalars(e, thermo);
+      return 0;
+   });
+}
+
+int mgpu_md_steps(mgpu_engine *e, int nsteps, int init_h0, int use_graph, double *trace, double pe[2],
+                  double stress[9]) {
+   return guarded(e, [&] {
+      if (!e->have_dyn) return fail(e, MGPU_E_STATE, "mgpu_set_dynamics not called");
+      if (e->n_full == 0) return fail(e, MGPU_E_STATE, "mgpu_upload_state not called");
+      if (nsteps < 0) return fail(e, MGPU_E_ARG, "nsteps < 0");
+      if (nsteps == 0) return 0;
+      HIPCHK(hipSetDevice(e->device));
+      md_reset_record(e, nsteps);
+      zero_async(e, e->d_iflags.p, sizeof(int) * 48);
+      md_step_launch(e, init_h0 != 0);
+      int done = 1;
+      if (use_graph && nsteps > 1) {
+         if (!e->graph_exec || e->graph_version != e->geom_version || e->graph_precision != e->precision ||
+             e->graph_epoch != g_alloc_epoch) {
+            if (e->graph_exec) HIPCHK(hipGraphExecDestroy(e->graph_exec));
+            if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
+            e->graph_exec = nullptr;
+            e->graph = nullptr;
+            // the eager step above made every buffer; capture one more step
+            unsigned long epoch = g_alloc_epoch;
+            e->capturing = true;
+            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            try {
+               md_step_launch(e, false);
+            } catch (...) {
+               hipGraph_t g = nullptr;
+               (void)hipStreamEndCapture(e->stream, &g);
+               if (g) (void)hipGraphDestroy(g);
+               e->capturing = false;
+               throw;
+            }
+            HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
+            e->capturing = false;
+            if (epoch != g_alloc_epoch) throw StatusError{MGPU_E_STATE, "allocation during graph capture"};
+            HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+            e->graph_version = e->geom_version;
+            e->graph_precision = e->precision;
+            e->graph_epoch = g_alloc_epoch;
+         }
+         for (; done < nsteps; done++) HIPCHK(hipGraphLaunch(e->graph_exec, e->stream));
+      }
+      for (; done < nsteps; done++) md_step_launch(e, false);
+      if (trace)
+         HIPCHK(hipMemcpyAsync(trace, e->d_trace.p, sizeof(double) * md_trace_width() * nsteps,
+                               hipMemcpyDeviceToHost, e->stream));
+      double r[14];
+      int fl[41];
+      HIPCHK(hipMemcpyAsync(r, e->d_out.p + OUT_RES, sizeof r, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(fl, e->d_iflags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      handle_flags(e, fl);
+      if (pe) { pe[0] = r[0]; pe[1] = r[1]; }
+      if (stress) for (int q = 0; q < 9; q++) stress[q] = r[5 + q];
+      return 0;
+   });
+}
+
+int mgpu_download_dynamics(mgpu_engine *e, double *cm, double *quat, double *mom, double *amom,
+                           double thermo[3]) {
+   return guarded(e, [&] {
+      if (!e->have_dyn) return fail(e, MGPU_E_STATE, "mgpu_set_dynamics not called");
+      HIPCHK(hipSetDevice(e->device));
+      if (cm) HIPCHK(hipMemcpyAsync(cm, e->d_cm.p, sizeof(double) * 3 * e->nmols, hipMemcpyDeviceToHost, e->stream));
+      if (quat && e->nmols_r)
+         HIPCHK(hipMemcpyAsync(quat, e->d_quat.p, sizeof(double) * 4 * e->nmols_r, hipMemcpyDeviceToHost, e->stream));
+      if (mom) HIPCHK(hipMemcpyAsync(mom, e->d_mom.p, sizeof(double) * 3 * e->nmols, hipMemcpyDeviceToHost, e->stream));
+      if (amom && e->nmols_r)
+         HIPCHK(hipMemcpyAsync(amom, e->d_amom.p, sizeof(double) * 4 * e->nmols_r, hipMemcpyDeviceToHost, e->stream));
+      double th[3];
+      md_get_scalars(e, th);
+      if (thermo) for (int k = 0; k < 3; k++) thermo[k] = th[k];
+      return 0;
+   });
+}
+
 int mgpu_last_info(const mgpu_engine *e, mgpu_fc_info *fc, mgpu_ew_info *ew) {
    if (!e) return MGPU_E_ARG;
    if (fc) *fc = e->last_fc;
@@ -844,7 +950,7 @@ int mgpu_cells(mgpu_engine *e, const double h[9], const double *site, int ld, co
       upload_sites(e, site, ld);
       if (e->molpbc && cm)
          HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
-      zero_async(e, e->d_iflags.p, sizeof(int) * 40);
+      zero_async(e, e->d_iflags.p, sizeof(int) * 48);
       run_binning(e, false);
       if (dims3) { dims3[0] = e->nx; dims3[1] = e->ny; dims3[2] = e->nz; }
       if (entry_cell)
@@ -871,7 +977,7 @@ int mgpu_pairlist(mgpu_engine *e, const double h[9], const double *site, int ld,
       upload_sites(e, site, ld);
       if (e->molpbc && cm)
          HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
-      zero_async(e, e->d_iflags.p, sizeof(int) * 40);
+      zero_async(e, e->d_iflags.p, sizeof(int) * 48);
       run_binning(e, false);
       sync_flags(e);
       std::vector<int> off;

@@ -32,6 +32,10 @@ struct StatusError {
    std::string msg;
 };
 
+// Counts device (re)allocations of engine buffers: a captured graph holds raw
+// pointers and is valid only while this is unchanged.
+extern unsigned long g_alloc_epoch;
+
 // Device buffer owned by the engine (grows, never shrinks).
 template <class T> struct DBuf {
    T *p = nullptr;
@@ -43,6 +47,7 @@ template <class T> struct DBuf {
       size_t bytes = (m ? m : 1) * sizeof(T);
       HIPCHK(hipMalloc((void **)&p, bytes));
       n = m;
+      g_alloc_epoch++;
    }
    void free() {
       if (p) (void)hipFree(p);
@@ -168,6 +173,7 @@ struct mgpu_engine {
    hipGraph_t graph = nullptr;
    hipGraphExec_t graph_exec = nullptr;
    int graph_version = -1, graph_precision = -1;
+   unsigned long graph_epoch = 0;
    double dyn_step = 0, dyn_ttmass = 0, dyn_pmass = 0, dyn_gkt = 0;
    int dyn_const_temp = 0, dyn_nosym = 0, dyn_saxis = 0;
    std::vector<double> spec_mass, spec_inertia;
@@ -212,6 +218,13 @@ void run_ewald_mixed(mgpu_engine *e);       // -> d_out[OUT_RECIP..]
 // correction and the assembly of pe, dipole and stress into d_out[OUT_RES..].
 void run_glue(mgpu_engine *e);
 void eval_forces_device(mgpu_engine *e, bool clear_flags);
+// molecular dynamics (mgpu_dynamics.hip)
+void md_setup(mgpu_engine *e, const mgpu_dynamics *d);
+void md_set_scalars(mgpu_engine *e, const double thermo[3]);
+void md_get_scalars(mgpu_engine *e, double thermo[3]);
+void md_reset_record(mgpu_engine *e, int cap);
+void md_step_launch(mgpu_engine *e, bool init_h0);
+int md_trace_width();
 // Fill with zeros by a kernel (safe inside a captured graph).
 void zero_async(mgpu_engine *e, void *p, size_t bytes);
 void device_scan_int(mgpu_engine *e, const int *in, int *out, int n);  // exclusive, out[n]=total

@@ -21,7 +21,8 @@ _ip = C.POINTER(C.c_int)
 _lp = C.POINTER(C.c_long)
 
 ERRORS = {1: "ARG", 2: "HIP", 3: "BIN_RANGE", 4: "BIN_ROUND", 5: "CUTOFF", 6: "TONAB",
-          7: "MULTIFW", 8: "PTYPE", 9: "STATE", 10: "CAPACITY"}
+          7: "MULTIFW", 8: "PTYPE", 9: "STATE", 10: "CAPACITY", 11: "QNORM"}
+TRACE_W = 8
 
 
 class EngineError(RuntimeError):
@@ -46,6 +47,12 @@ class FcInfo(C.Structure):
                 ("n_too_close", C.c_int), ("too_close_i", C.c_int * 16),
                 ("too_close_j", C.c_int * 16), ("n_list_entries", C.c_long),
                 ("max_cell_sites", C.c_int), ("eintra", C.c_double)]
+
+
+class MgpuDynamics(C.Structure):
+    _fields_ = [("step", C.c_double), ("ttmass", C.c_double), ("pmass", C.c_double),
+                ("gkt", C.c_double), ("const_temp", C.c_int), ("nosymmetric_rot", C.c_int),
+                ("spec_mass", _dp), ("spec_inertia", _dp)]
 
 
 class EwInfo(C.Structure):
@@ -181,6 +188,45 @@ class ForceEngine:
         fc, ew = FcInfo(), EwInfo()
         lib().mgpu_last_info(self.h_, C.byref(fc), C.byref(ew))
         return fc, ew
+
+    # -------------------------------------------------- molecular dynamics
+    def set_dynamics(self, dyn):
+        """dyn: moldy_amd.system.Dynamics (program units)."""
+        from moldy_amd import units
+        s = self.sys
+        mass, inertia = s.species_mass_inertia()
+        self._dyn_arrays = (np.ascontiguousarray(mass), np.ascontiguousarray(inertia))
+        self.dyn = MgpuDynamics(dyn.step, dyn.ttmass, dyn.pmass,
+                                s.degrees_of_freedom() * units.KB * dyn.temp,
+                                int(dyn.const_temp), int(dyn.nosymmetric_rot),
+                                _p(self._dyn_arrays[0]), _p(self._dyn_arrays[1]))
+        self._chk(lib().mgpu_set_dynamics(self.h_, C.byref(self.dyn)))
+
+    def upload_momenta(self, mom, amom, ts=1.0, tsmom=0.0, H_0=0.0):
+        mom = np.ascontiguousarray(mom, np.float64)
+        amom = np.ascontiguousarray(amom, np.float64)
+        th = np.array([ts, tsmom, H_0])
+        self._chk(lib().mgpu_upload_momenta(self.h_, _p(mom), _p(amom), _p(th)))
+
+    def md_steps(self, nsteps, init_H_0=False, use_graph=True):
+        """Runs nsteps do_steps; returns (trace[nsteps][8], pe[2], stress[3,3])."""
+        tr = np.zeros((max(nsteps, 1), TRACE_W))
+        pe = np.zeros(2)
+        st = np.zeros(9)
+        self._chk(lib().mgpu_md_steps(self.h_, int(nsteps), int(init_H_0), int(use_graph),
+                                      _p(tr), _p(pe), _p(st)))
+        return tr[:nsteps], pe, st.reshape(3, 3)
+
+    def download_dynamics(self):
+        s = self.sys
+        cm = np.zeros((s.nmols, 3))
+        q = np.zeros((max(s.nmols_r, 1), 4))
+        mom = np.zeros((s.nmols, 3))
+        amom = np.zeros((max(s.nmols_r, 1), 4))
+        th = np.zeros(3)
+        self._chk(lib().mgpu_download_dynamics(self.h_, _p(cm), _p(q), _p(mom), _p(amom), _p(th)))
+        return dict(c_of_m=cm, quat=q[:s.nmols_r], mom=mom, amom=amom[:s.nmols_r], ts=th[0],
+                    tsmom=th[1], H_0=th[2])
 
     # ------------------------------------------------ cells and pair lists
     def cells(self, h, xyz, c_of_m=None):

@@ -48,6 +48,17 @@ class Control:
 
 
 @dataclasses.dataclass
+class Dynamics:
+    """The contr_mt fields do_step reads (src/accel.c:626-827), program units."""
+    step: float = 0.001            # ps
+    temp: float = 0.0              # K, thermostat target
+    const_temp: int = 0            # 1: Nose-Poincare thermostat
+    ttmass: float = 100.0
+    pmass: float = 100.0
+    nosymmetric_rot: bool = False  # True: leapf_quat_a splitting
+
+
+@dataclasses.dataclass
 class MoldySystem:
     species: List[Species]
     site_mass: np.ndarray           # [max_id]
@@ -127,6 +138,54 @@ class MoldySystem:
             quat=np.ascontiguousarray(self.quat.reshape(-1, 4), np.float64)
             if self.nmols_r else np.zeros((1, 4)),
         )
+
+    # ------------------------------------------------------------ dynamics
+    def species_mass_inertia(self):
+        """Molecular mass and principal moments of inertia of every species
+        (src/startup.c:486-560 computes them from the site masses; the
+        p_f_sites here already sit in the principal frame)."""
+        mass = np.zeros(self.nspecies)
+        inertia = np.zeros((self.nspecies, 3))
+        for k, s in enumerate(self.species):
+            m = self.site_mass[np.asarray(s.site_ids)]
+            p = np.asarray(s.p_f_sites, np.float64).reshape(-1, 3)
+            mass[k] = m.sum()
+            if s.rdof > 0:
+                I = np.einsum("i,ij,ik->jk", m, p, p)
+                I = np.eye(3) * np.trace(I) - I
+                off = np.abs(I - np.diag(np.diag(I))).max()
+                if off > 1e-9 * max(np.abs(I).max(), 1e-300):
+                    raise ValueError(f"species {s.name}: sites are not in the principal frame")
+                inertia[k] = np.diag(I)
+        return mass, inertia
+
+    def degrees_of_freedom(self):
+        """sys->d_of_f, src/startup.c:484 and 570."""
+        return float(sum((0 if s.framework else 3 * s.nmols) + s.rdof * s.nmols
+                         for s in self.species))
+
+    def thermal_momenta(self, temp, seed=1):
+        """Maxwell-Boltzmann scaled momenta mom = h^T p and principal angular
+        momenta at temperature temp (K), from a counter-based generator; the
+        total momentum is removed.  Frameworks get zero momentum."""
+        rng = _rng(seed)
+        mass, inertia = self.species_mass_inertia()
+        kT = units.KB * temp
+        mom = np.zeros((self.nmols, 3))
+        amom = np.zeros((max(self.nmols_r, 1), 4))
+        im = iq = 0
+        for k, s in enumerate(self.species):
+            if not s.framework:
+                mom[im:im + s.nmols] = rng.standard_normal((s.nmols, 3)) * math.sqrt(mass[k] * kT)
+            if s.rdof > 0:
+                sd = np.sqrt(np.maximum(inertia[k], 0.0) * kT)
+                amom[iq:iq + s.nmols, 1:] = rng.standard_normal((s.nmols, 3)) * sd
+                iq += s.nmols
+            im += s.nmols
+        moving = np.concatenate([np.full(s.nmols, not s.framework) for s in self.species])
+        if moving.any():
+            mom[moving] -= mom[moving].mean(0)
+        return mom @ self.h, amom[:max(self.nmols_r, 1)]
 
     def copy(self):
         return dataclasses.replace(

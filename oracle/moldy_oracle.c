@@ -1313,3 +1313,387 @@ out:
    free(cmr);
    return rc;
 }
+
+/* ====================================================================== */
+/* Molecular dynamics step: the leapfrog splitting of src/accel.c:626-827  */
+/* with the symplectic rotation of src/leapfrog.c.  No barostat.           */
+/* ====================================================================== */
+
+#define ORC_INERTIA_MIN 1.0e-14
+#define ORC_QTOL 1.0e-4
+#define ORC_DBL_MIN 2.2250738585072014e-308
+
+/* q_mul / q_conj_mul, src/quaterns.c:38-98 (r may alias p or q) */
+static void orc_qmul(const double *p, const double *q, double *r)
+{
+   double p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+   double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+   r[0] = p0 * q0 - p1 * q1 - p2 * q2 - p3 * q3;
+   r[1] = p1 * q0 + p0 * q1 - p3 * q2 + p2 * q3;
+   r[2] = p2 * q0 + p3 * q1 + p0 * q2 - p1 * q3;
+   r[3] = p3 * q0 - p2 * q1 + p1 * q2 + p0 * q3;
+}
+static void orc_qconjmul(const double *p, const double *q, double *r)
+{
+   double pc[4];
+   pc[0] = p[0]; pc[1] = -p[1]; pc[2] = -p[2]; pc[3] = -p[3];
+   orc_qmul(pc, q, r);
+}
+
+/* rot_substep, src/leapfrog.c:238-244: amom <- rot* amom rot, quat <- quat rot */
+static void orc_rot_substep(const double *rot, double *amom, double *quat)
+{
+   orc_qconjmul(rot, amom, amom);
+   orc_qmul(amom, rot, amom);
+   orc_qmul(quat, rot, quat);
+}
+
+/* make_rot, src/leapfrog.c:193-208 */
+static void orc_make_rot(double step, int axis, const double *amom, double rinertia, double *rot)
+{
+   double angle = 0.5 * step * rinertia * amom[axis + 1];
+   rot[0] = cos(angle);
+   rot[1] = rot[2] = rot[3] = 0.0;
+   rot[axis + 1] = sin(angle);
+}
+
+/* make_rot_amom, src/leapfrog.c:213-231 */
+static void orc_make_rot_amom(double step, const double *amom, double rinertia, double *rot)
+{
+   double samom = sqrt(amom[1] * amom[1] + amom[2] * amom[2] + amom[3] * amom[3]);
+   double ramom = 1.0 / (samom + (8 * ORC_DBL_MIN));
+   double angle = 0.5 * step * rinertia * samom, sa = sin(angle);
+   rot[0] = cos(angle);
+   rot[1] = sa * ramom * amom[1];
+   rot[2] = sa * ramom * amom[2];
+   rot[3] = sa * ramom * amom[3];
+}
+
+/* rot_ke, src/algorith.c:244-259 */
+static double orc_rot_ke(const double *amom, int nmols, double s, const double *inertia)
+{
+   double ke = 0.0, t;
+   int i, m;
+   for (i = 0; i < 3; i++)
+      if (inertia[i] > ORC_INERTIA_MIN) {
+         t = 0.0;
+         for (m = 0; m < nmols; m++)
+            t += amom[4 * m + i + 1] * amom[4 * m + i + 1];
+         ke += t / inertia[i];
+      }
+   return 0.5 * ke / (s * s);
+}
+
+/* trans_ke, src/algorith.c:221-239: real momenta (h^-1)^T mom */
+static double orc_trans_ke(const double h[9], const double *mom, int nmols, double s, double mass)
+{
+   double hi[9], ke = 0.0, p[3];
+   int m, i;
+   orc_invert(h, hi);
+   for (m = 0; m < nmols; m++) {
+      const double *q = mom + 3 * m;
+      for (i = 0; i < 3; i++)
+         p[i] = hi[i] * q[0] + hi[3 + i] * q[1] + hi[6 + i] * q[2];
+      ke += p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+   }
+   return ke / (2.0 * mass * s * s);
+}
+
+static void orc_rinertia(const double *inertia, double *rinertia)
+{
+   int i;
+   for (i = 0; i < 3; i++)
+      if (inertia[i] / (inertia[(i + 1) % 3] + inertia[(i + 2) % 3]) < ORC_INERTIA_MIN)
+         rinertia[i] = 0.0;
+      else
+         rinertia[i] = 1.0 / inertia[i];
+}
+
+/* Near-symmetry axis of leapf_quat_b, src/leapfrog.c:275-290.  The reference
+ * keeps the first call's choice in a static for every later species; the
+ * first call is made for the first species with rotational freedom. */
+int orc_symmetry_axis(const double *inertia)
+{
+   double rin[3], idmin = 1.0e37, idiff;
+   int i, saxis = 0;
+   orc_rinertia(inertia, rin);
+   for (i = 0; i < 3; i++) {
+      idiff = fabs(rin[(i + 1) % 3] - rin[(i + 2) % 3]);
+      if (idiff < idmin) {
+         idmin = idiff;
+         saxis = i;
+      }
+   }
+   return saxis;
+}
+
+/* normalise, src/leapfrog.c:96-113 */
+static int orc_normalise(double *quat, int n, int *bad)
+{
+   int i, j;
+   double norm;
+   for (i = 0; i < n; i++) {
+      norm = 0.0;
+      for (j = 0; j < 4; j++)
+         norm += quat[4 * i + j] * quat[4 * i + j];
+      norm = sqrt(norm);
+      if (fabs(norm - 1.0) > ORC_QTOL) {
+         *bad = i;
+         return ORC_ERR_QNORM;
+      }
+      for (j = 0; j < 4; j++)
+         quat[4 * i + j] /= norm;
+   }
+   return ORC_OK;
+}
+
+/* leapf_quat_b, src/leapfrog.c:255-338 */
+static int orc_leapf_quat_b(const orc_dyn *d, double step, double *quat, double *amom,
+                            const double *inertia, double *smom, double ts, int nmols,
+                            int saxis, int *bad)
+{
+   double rin[3], rot[4], stepdts = step / ts;
+   int o1 = (saxis + 1) % 3, o2 = (saxis + 2) % 3, m;
+   orc_rinertia(inertia, rin);
+   for (m = 0; m < nmols; m++) {
+      orc_make_rot(0.5 * stepdts, o1, amom + 4 * m, rin[o1] - rin[o2], rot);
+      orc_rot_substep(rot, amom + 4 * m, quat + 4 * m);
+   }
+   if (d->const_temp)
+      *smom += 0.5 * step * orc_rot_ke(amom, nmols, ts, inertia);
+   for (m = 0; m < nmols; m++) {
+      orc_make_rot(stepdts, saxis, amom + 4 * m, rin[saxis] - rin[o2], rot);
+      orc_rot_substep(rot, amom + 4 * m, quat + 4 * m);
+      orc_make_rot_amom(stepdts, amom + 4 * m, rin[o2], rot);
+      orc_qmul(quat + 4 * m, rot, quat + 4 * m);
+   }
+   if (d->const_temp)
+      *smom += 0.5 * step * orc_rot_ke(amom, nmols, ts, inertia);
+   for (m = 0; m < nmols; m++) {
+      orc_make_rot(0.5 * stepdts, o1, amom + 4 * m, rin[o1] - rin[o2], rot);
+      orc_rot_substep(rot, amom + 4 * m, quat + 4 * m);
+   }
+   return orc_normalise(quat, nmols, bad);
+}
+
+static double orc_axis_sumsq(const double *amom, int nmols, int axis)
+{
+   double t = 0.0;
+   int m;
+   for (m = 0; m < nmols; m++)
+      t += amom[4 * m + axis + 1] * amom[4 * m + axis + 1];
+   return t;
+}
+
+/* leapf_quat_a, src/leapfrog.c:345-392 */
+static int orc_leapf_quat_a(const orc_dyn *d, double step, double *quat, double *amom,
+                            const double *inertia, double *smom, double ts, int nmols, int *bad)
+{
+   double rin[3], rot[4], stepdts = step / ts, stepdts2 = step / (ts * ts);
+   static const int axis[5] = {0, 1, 2, 1, 0};
+   static const double frac[5] = {0.5, 0.5, 1.0, 0.5, 0.5};
+   int k, m;
+   orc_rinertia(inertia, rin);
+   for (k = 0; k < 5; k++) {
+      for (m = 0; m < nmols; m++) {
+         orc_make_rot(frac[k] * stepdts, axis[k], amom + 4 * m, rin[axis[k]], rot);
+         orc_rot_substep(rot, amom + 4 * m, quat + 4 * m);
+      }
+      if (d->const_temp) {
+         if (k == 0) {
+            *smom += 0.25 * stepdts2 * rin[0] * orc_axis_sumsq(amom, nmols, 0);
+            *smom += 0.25 * stepdts2 * rin[1] * orc_axis_sumsq(amom, nmols, 1);
+         } else if (k == 1) {
+            *smom += 0.5 * stepdts2 * rin[2] * orc_axis_sumsq(amom, nmols, 2);
+         } else if (k == 3) {
+            *smom += 0.25 * stepdts2 * rin[1] * orc_axis_sumsq(amom, nmols, 1);
+            *smom += 0.25 * stepdts2 * rin[0] * orc_axis_sumsq(amom, nmols, 0);
+         }
+      }
+   }
+   return orc_normalise(quat, nmols, bad);
+}
+
+/* leapf_all_coords, src/accel.c:360-372, with leapf_com (leapfrog.c:133-146) */
+static int orc_leapf_all_coords(const orc_sys *s, const orc_dyn *d, const double h[9],
+                                double step, orc_md_state *st, int saxis, int *bad)
+{
+   double ht[9], G[9], Gi[9], c;
+   int i, j, k, sp, m, imol = 0, iq = 0, rc;
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++)
+         ht[3 * i + j] = h[3 * j + i];
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++) {
+         G[3 * i + j] = 0.0;
+         for (k = 0; k < 3; k++)
+            G[3 * i + j] += ht[3 * i + k] * h[3 * k + j];
+      }
+   orc_invert(G, Gi);
+   for (sp = 0; sp < s->nspecies; sp++) {
+      int nm = s->spec_nmols[sp];
+      double A[9];
+      c = step / (d->spec_mass[sp] * st->ts);
+      for (i = 0; i < 9; i++)
+         A[i] = c * Gi[i];
+      for (m = imol; m < imol + nm; m++) {
+         double *r = st->c_of_m + 3 * m;
+         const double *p = st->mom + 3 * m;
+         double y0 = r[0] + A[0] * p[0] + A[1] * p[1] + A[2] * p[2];
+         double y1 = r[1] + A[3] * p[0] + A[4] * p[1] + A[5] * p[2];
+         double y2 = r[2] + A[6] * p[0] + A[7] * p[1] + A[8] * p[2];
+         r[0] = y0; r[1] = y1; r[2] = y2;
+         /* escape, leapfrog.c:117-128 */
+         r[0] -= floor(r[0] + 0.5);
+         r[1] -= floor(r[1] + 0.5);
+         r[2] -= floor(r[2] + 0.5);
+      }
+      if (s->spec_rdof[sp] > 0) {
+         const double *I = d->spec_inertia + 3 * sp;
+         if (d->nosymmetric_rot)
+            rc = orc_leapf_quat_a(d, step, st->quat + 4 * iq, st->amom + 4 * iq, I, &st->tsmom,
+                                  st->ts, nm, bad);
+         else
+            rc = orc_leapf_quat_b(d, step, st->quat + 4 * iq, st->amom + 4 * iq, I, &st->tsmom,
+                                  st->ts, nm, saxis, bad);
+         if (rc) {
+            *bad += iq;
+            return rc;
+         }
+         iq += nm;
+      }
+      imol += nm;
+   }
+   return ORC_OK;
+}
+
+/* leapf_all_momenta, src/accel.c:376-388, with leapf_mom / leapf_amom */
+static void orc_leapf_all_momenta(const orc_sys *s, const double h[9], double step,
+                                  orc_md_state *st, const double *force, const double *torque)
+{
+   double A[9], c = step * st->ts;
+   int i, j, m, iq = 0, imol = 0, sp;
+   for (i = 0; i < 3; i++)
+      for (j = 0; j < 3; j++)
+         A[3 * i + j] = c * h[3 * j + i];
+   for (sp = 0; sp < s->nspecies; sp++) {
+      int nm = s->spec_nmols[sp];
+      for (m = imol; m < imol + nm; m++) {
+         double *p = st->mom + 3 * m;
+         const double *f = force + 3 * m;
+         double y0 = p[0] + A[0] * f[0] + A[1] * f[1] + A[2] * f[2];
+         double y1 = p[1] + A[3] * f[0] + A[4] * f[1] + A[5] * f[2];
+         double y2 = p[2] + A[6] * f[0] + A[7] * f[1] + A[8] * f[2];
+         p[0] = y0; p[1] = y1; p[2] = y2;
+      }
+      if (s->spec_rdof[sp] > 0) {
+         for (m = iq; m < iq + nm; m++) {
+            st->amom[4 * m + 1] += c * torque[3 * m];
+            st->amom[4 * m + 2] += c * torque[3 * m + 1];
+            st->amom[4 * m + 3] += c * torque[3 * m + 2];
+         }
+         iq += nm;
+      }
+      imol += nm;
+   }
+}
+
+/* tot_ke, src/accel.c:333-345 */
+static void orc_tot_ke(const orc_sys *s, const orc_dyn *d, const double h[9],
+                       const orc_md_state *st, double *ke_t, double *ke_r)
+{
+   int sp, imol = 0, iq = 0;
+   *ke_t = 0.0;
+   *ke_r = 0.0;
+   for (sp = 0; sp < s->nspecies; sp++) {
+      int nm = s->spec_nmols[sp];
+      *ke_t += orc_trans_ke(h, st->mom + 3 * imol, nm, st->ts, d->spec_mass[sp]);
+      if (s->spec_rdof[sp] > 0) {
+         *ke_r += orc_rot_ke(st->amom + 4 * iq, nm, st->ts, d->spec_inertia + 3 * sp);
+         iq += nm;
+      }
+      imol += nm;
+   }
+}
+
+/* leapf_nose_therm, src/leapfrog.c:468-474 */
+static void orc_nose_therm(double step, double mass, double *s, double *smom)
+{
+   double scale = (1.0 + *smom * step / (2.0 * mass));
+   *smom /= scale;
+   *s *= scale * scale;
+}
+
+int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_state *st,
+                int init_H_0, double pe[2], double stress[9], double rec[7])
+{
+   int nm = s->nmols, sp, rc, bad = -1, imol, saxis = 0, m;
+   double step = d->step, ke_t, ke_r, dip[3];
+   double *force = (double *)calloc((size_t)nm * 6 + 6, sizeof(double)), *torque;
+   if (!force)
+      return ORC_ERR_ALLOC;
+   torque = force + 3 * (size_t)nm + 3;
+   for (sp = 0; sp < s->nspecies; sp++)
+      if (s->spec_rdof[sp] > 0) {
+         saxis = orc_symmetry_axis(d->spec_inertia + 3 * sp);
+         break;
+      }
+   /* H1 half step */
+   if (d->const_temp) {
+      orc_nose_therm(0.5 * step, d->ttmass, &st->ts, &st->tsmom);
+      st->tsmom -= 0.5 * step * d->gkt * (1.0 + log(st->ts));
+   }
+   /* H3 half step */
+   if (d->const_temp) {
+      orc_tot_ke(s, d, h, st, &ke_t, &ke_r);
+      st->tsmom += 0.5 * step * ke_t;
+   }
+   rc = orc_leapf_all_coords(s, d, h, 0.5 * step, st, saxis, &bad);
+   if (rc)
+      goto out;
+   /* H4 full step */
+   rc = orc_eval_forces(s, h, st->c_of_m, st->quat, pe, dip, stress, force, torque, NULL, NULL,
+                        NULL, NULL);
+   if (rc)
+      goto out;
+   orc_leapf_all_momenta(s, h, 0.5 * step, st, force, torque);
+   orc_tot_ke(s, d, h, st, &ke_t, &ke_r);
+   if (init_H_0)
+      st->H_0 = (ke_t + ke_r) + (pe[0] + pe[1]) + st->tsmom * st->tsmom / (2.0 * d->ttmass) +
+                d->gkt * log(st->ts) + 0.5 * 0.0 / d->pmass;
+   if (rec) {
+      rec[0] = ke_t;
+      rec[1] = ke_r;
+      rec[2] = pe[0];
+      rec[3] = pe[1];
+      rec[4] = st->ts;
+      rec[5] = st->tsmom;
+      rec[6] = st->H_0;
+   }
+   orc_leapf_all_momenta(s, h, 0.5 * step, st, force, torque);
+   /* frameworks do not move, src/accel.c:752-757 */
+   for (sp = 0, imol = 0; sp < s->nspecies; sp++) {
+      if (s->spec_framework[sp])
+         for (m = imol; m < imol + s->spec_nmols[sp]; m++)
+            st->mom[3 * m] = st->mom[3 * m + 1] = st->mom[3 * m + 2] = 0.0;
+      imol += s->spec_nmols[sp];
+   }
+   if (d->const_temp)
+      st->tsmom -= step * (pe[0] + pe[1] - st->H_0);
+   if (d->const_temp) {
+      orc_tot_ke(s, d, h, st, &ke_t, &ke_r);
+      st->tsmom += 0.5 * step * ke_t;
+   }
+   rc = orc_leapf_all_coords(s, d, h, 0.5 * step, st, saxis, &bad);
+   if (rc)
+      goto out;
+   if (d->const_temp) {
+      st->tsmom -= 0.5 * step * d->gkt * (1.0 + log(st->ts));
+      orc_nose_therm(0.5 * step, d->ttmass, &st->ts, &st->tsmom);
+   }
+out:
+   if (rc == ORC_ERR_QNORM)
+      st->bad_quat = bad;
+   free(force);
+   return rc;
+}

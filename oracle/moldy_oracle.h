@@ -33,7 +33,8 @@ enum {
    ORC_ERR_MULTIFW   = 5,     /* multiple frameworks                force.c:1219*/
    ORC_ERR_PTYPE     = 6,     /* UNKPTY                             kernel.c:186*/
    ORC_ERR_MAP       = 7,     /* strict map index outside the map (UB in ref) */
-   ORC_ERR_ALLOC     = 8
+   ORC_ERR_ALLOC     = 8,
+   ORC_ERR_QNORM     = 9      /* QNORM2 fatal                       leapfrog.c:104 */
 };
 
 /*
@@ -151,6 +152,35 @@ void orc_make_sites(const orc_sys *s, const double h[9], const double *c_of_m_fr
                     double *z);
 /* distant_const (src/accel.c:293-329) */
 double orc_distant_const(const orc_sys *s, double cutoff, int iflag);
+
+/* ---- molecular dynamics: do_step (src/accel.c:626-827) with the leapfrog
+ * splitting of src/leapfrog.c, at constant h (no barostat) ---- */
+typedef struct {
+   double step;                  /* control.step                              */
+   double ttmass;                /* control.ttmass (thermostat mass)          */
+   double pmass;                 /* control.pmass (enters H_0 via ke_cell)    */
+   double gkt;                   /* sys->d_of_f * kB * control.temp           */
+   int const_temp;               /* 0, or 1 = Nose-Poincare thermostat        */
+   int nosymmetric_rot;          /* control.nosymmetric_rot: leapf_quat_a     */
+   const double *spec_mass;      /* [nspecies]                                */
+   const double *spec_inertia;   /* [nspecies][3] principal moments           */
+} orc_dyn;
+
+typedef struct {
+   double *c_of_m;               /* [nmols][3] scaled centre-of-mass coords   */
+   double *quat;                 /* [nmols_r][4]                              */
+   double *mom;                  /* [nmols][3] scaled momenta                 */
+   double *amom;                 /* [nmols_r][4] principal angular momenta    */
+   double ts, tsmom, H_0;        /* thermostat s, its momentum, H_0           */
+   int bad_quat;                 /* molecule of a QNORM failure               */
+} orc_md_state;
+
+/* One do_step.  pe[2] and stress[9] of the force evaluation; rec[7] (optional)
+ * = translational KE, rotational KE, pe[0], pe[1], s, p_s and H_0 at the
+ * point where do_step evaluates H_0 (after the first momentum half step). */
+int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_state *st,
+                int init_H_0, double pe[2], double stress[9], double rec[7]);
+int orc_symmetry_axis(const double *inertia);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,8 @@ import subprocess
 
 import numpy as np
 
+from moldy_amd import units
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 
@@ -41,6 +43,19 @@ class EwInfo(C.Structure):
     _fields_ = [("hmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int), ("nhkl", C.c_int),
                 ("nsitesxf", C.c_int), ("self_energy", C.c_double),
                 ("sheet_energy", C.c_double)]
+
+
+class OrcDyn(C.Structure):
+    _fields_ = [("step", C.c_double), ("ttmass", C.c_double), ("pmass", C.c_double),
+                ("gkt", C.c_double), ("const_temp", C.c_int), ("nosymmetric_rot", C.c_int),
+                ("spec_mass", C.POINTER(C.c_double)), ("spec_inertia", C.POINTER(C.c_double))]
+
+
+class OrcMdState(C.Structure):
+    _fields_ = [("c_of_m", C.POINTER(C.c_double)), ("quat", C.POINTER(C.c_double)),
+                ("mom", C.POINTER(C.c_double)), ("amom", C.POINTER(C.c_double)),
+                ("ts", C.c_double), ("tsmom", C.c_double), ("H_0", C.c_double),
+                ("bad_quat", C.c_int)]
 
 
 def _build():
@@ -218,6 +233,31 @@ class Oracle:
                                    _p(sf), C.byref(fci), C.byref(ewi))
         return dict(rc=rc, pe=pe, dip=dip, stress=stress.reshape(3, 3), force=force,
                     torque=torque[:s.nmols_r], site_xyz=sxyz, site_force=sf, fc=fci, ew=ewi)
+
+    def md(self, dyn, mom, amom, c_of_m=None, quat=None, ts=1.0, tsmom=0.0, H_0=0.0):
+        """Oracle MD run: returns a callable step(init_H_0) -> (rc, pe, stress,
+        rec) that advances its own copy of the state, and the state dict."""
+        s = self.sys
+        mass, inertia = s.species_mass_inertia()
+        st = dict(c_of_m=np.array(s.c_of_m if c_of_m is None else c_of_m, np.float64),
+                  quat=np.array(self.f["quat"] if quat is None else quat, np.float64).reshape(-1, 4),
+                  mom=np.array(mom, np.float64).reshape(-1, 3),
+                  amom=np.array(amom, np.float64).reshape(-1, 4),
+                  mass=np.ascontiguousarray(mass), inertia=np.ascontiguousarray(inertia))
+        d = OrcDyn(dyn.step, dyn.ttmass, dyn.pmass,
+                   s.degrees_of_freedom() * units.KB * dyn.temp, int(dyn.const_temp),
+                   int(dyn.nosymmetric_rot), _p(st["mass"]), _p(st["inertia"]))
+        ms = OrcMdState(_p(st["c_of_m"]), _p(st["quat"]), _p(st["mom"]), _p(st["amom"]),
+                        ts, tsmom, H_0, -1)
+        st["_keep"] = (d, ms)
+
+        def step(init_H_0=False):
+            pe, stress, rec = np.zeros(2), np.zeros(9), np.zeros(8)
+            rc = lib().orc_do_step(C.byref(self.cs), C.byref(d), _p(self.h), C.byref(ms),
+                                   int(init_H_0), _p(pe), _p(stress), _p(rec))
+            st["ts"], st["tsmom"], st["H_0"], st["bad_quat"] = ms.ts, ms.tsmom, ms.H_0, ms.bad_quat
+            return rc, pe, stress.reshape(3, 3), rec
+        return step, st
 
     def distant_const(self, iflag):
         return lib().orc_distant_const(C.byref(self.cs), C.c_double(self.sys.control.cutoff),
