@@ -68,6 +68,57 @@ def time_gpu(eng, steps, warmup):
     return (t1 - t0) / steps, prof
 
 
+def equilibrate(s, T=300.0, nblocks=8, nper=150, dt=0.001, precision=1, seed=7):
+    """Relax the lattice start: short NVE blocks with the momenta rescaled to
+    T after each (translational and rotational parts separately)."""
+    from moldy_amd import engine as E, system, units
+    e = E.ForceEngine(s, precision=precision)
+    e.set_dynamics(system.Dynamics(step=dt))
+    e.upload_state()
+    mom, amom = s.thermal_momenta(T, seed=seed)
+    e.upload_momenta(mom, amom)
+    kT = units.KB * T
+    for _ in range(nblocks):
+        tr, _, _ = e.md_steps(nper, init_H_0=True)
+        d = e.download_dynamics()
+        kt, kr = tr[-1, 0], tr[-1, 1]
+        ft = math.sqrt(1.5 * s.nmols * kT / kt)
+        fr = math.sqrt(1.5 * s.nmols_r * kT / kr) if kr > 0 else 1.0
+        e.upload_momenta(d["mom"] * ft, d["amom"] * fr)
+    d = e.download_dynamics()
+    e.close()
+    return d
+
+
+def nve_run(s, start, precision, nsteps, dt=0.001, chunk=250):
+    """NVE from a common start: H = KE + PE at do_step's H_0 point per step,
+    its linear drift and the rms deviation about the fit, and the device
+    time per MD step (hipGraph-replayed steps, no host work between them)."""
+    from moldy_amd import engine as E, system, units
+    e = E.ForceEngine(s, precision=precision)
+    e.set_dynamics(system.Dynamics(step=dt))
+    e.upload_state(c_of_m=start["c_of_m"], quat=start["quat"])
+    e.upload_momenta(start["mom"], start["amom"])
+    e.md_steps(2, init_H_0=True)            # capture the graph outside the timing
+    e.upload_state(c_of_m=start["c_of_m"], quat=start["quat"])
+    e.upload_momenta(start["mom"], start["amom"])
+    trs = []
+    t0 = time.perf_counter()
+    for c in range(nsteps // chunk):
+        tr, _, _ = e.md_steps(chunk, init_H_0=(c == 0))
+        trs.append(tr)
+    t1 = time.perf_counter()
+    e.close()
+    tr = np.vstack(trs)
+    H = tr[:, :4].sum(1) * units.CONV_E / s.nmols           # kJ/mol per molecule
+    t_ps = np.arange(len(H)) * dt
+    fit = np.polyfit(t_ps, H, 1)
+    return dict(drift_kjmol_per_mol_per_ns=float(fit[0] * 1000.0),
+                rms_fluct_kjmol_per_mol=float(np.std(H - np.polyval(fit, t_ps))),
+                max_excursion_kjmol_per_mol=float(np.abs(H - H[0]).max()),
+                ms_per_md_step=1e3 * (t1 - t0) / len(H), steps=len(H), dt_fs=dt * 1000)
+
+
 def cpu_baseline(s, budget_s=25.0):
     """Oracle (single core) on the same configuration: force_calc in full,
     ewald over a k-space subset, extrapolated linearly in the k count."""
@@ -104,6 +155,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="17k-site variant (smoke only)")
+    ap.add_argument("--nve-steps", type=int, default=1000,
+                    help="NVE steps per precision for the drift comparison (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the force "
@@ -169,6 +222,17 @@ def main():
                                frac=ach / peak, traffic=None,
                                flops_per_launch=kind[1], avg_ms=prof[dom])
     res["kernel_ms"] = prof
+    eng.close()
+    if a.nve_steps > 0:
+        start = equilibrate(s)
+        nve = {}
+        for name, p in (("fp64", E.FP64), ("mixed", E.MIXED)):
+            nve[name] = nve_run(s, start, p, a.nve_steps)
+            print(f"[bench] nve {name}: {nve[name]}", file=sys.stderr, flush=True)
+        nve["start"] = "8 x 150 steps at 1 fs with momenta rescaled to 300 K after each block"
+        res["nve"] = nve
+        res["md_ns_per_day"] = {k: 86400.0 * 1e-6 * nve[k]["dt_fs"] / (nve[k]["ms_per_md_step"] * 1e-3)
+                                for k in ("fp64", "mixed")}
     if not a.no_cpu:
         cb = cpu_baseline(s)
         res["cpu_baseline"] = cb
