@@ -148,6 +148,17 @@ int mgpu_pairlist(mgpu_engine *eng, const double h[9], const double *site, int l
                   const double *c_of_m_frac, int strict, double cutoff, long *total,
                   long *list_start, int *nab, int *pbctrans, int *nfnab);
 
+/* ---- radial distribution functions (rdf_inner, src/force.c:1010-1103;
+ * rdf_accum, src/rdf.c:94-108; run from force_calc at src/force.c:1302-1313).
+ * Adds 1/density to rdf[block][bin] for every site pair closer than limit,
+ * found with the strict neighbour list at limit over force_calc's cells;
+ * block = init_rdf's (id_i, id_j) block order (src/rdf.c:71-89), so rdf is
+ * [(max_id-1)*max_id/2][nbins] floats and may be Moldy's rdf_ptr() array.
+ * Bins are exact: the pair distances are formed as the reference forms them
+ * and the float additions are rounded as its loop rounds them. */
+int mgpu_rdf(mgpu_engine *eng, const double h[9], const double *site, int ld,
+             const double *c_of_m_frac, double limit, int nbins, float *rdf);
+
 /* ---- timing helpers for measurement ---- */
 int mgpu_sync(mgpu_engine *eng);
 /* Average device time (ms) of the named kernel over the calls since

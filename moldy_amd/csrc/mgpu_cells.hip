@@ -369,21 +369,135 @@ void mgpu::run_pairlist_count(mgpu_engine *e, int, double, const int *offs, int 
 
 void mgpu::run_pairlist_fill(mgpu_engine *e, const int *, int noff, long total, int *nab,
                              int *pbctrans, int *nfnab) {
+   run_pairlist_fill_device(e, noff, total);
+   int nc = e->ncells;
+   if (nab) HIPCHK(hipMemcpyAsync(nab, e->d_pl_nab.p, sizeof(int) * total, hipMemcpyDeviceToHost, e->stream));
+   if (pbctrans)
+      HIPCHK(hipMemcpyAsync(pbctrans, e->d_pl_pbc.p, sizeof(int) * total, hipMemcpyDeviceToHost, e->stream));
+   if (nfnab)
+      HIPCHK(hipMemcpyAsync(nfnab, e->d_pl_nfnab.p, sizeof(int) * 2 * nc, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+}
+
+// Site neighbour lists of every cell into engine buffers (CSR over cells by
+// d_pl_start; counts from run_pairlist_count).
+void mgpu::run_pairlist_fill_device(mgpu_engine *e, int noff, long total) {
    ProfScope ps(e, "pairlist_fill");
    int nc = e->ncells;
-   mgpu::DBuf<int> dn, dp, df;
-   dn.reserve(total + 1);
-   dp.reserve(total + 1);
-   df.reserve(2 * (size_t)nc + 2);
+   e->d_pl_nab.reserve(total + 1);
+   e->d_pl_pbc.reserve(total + 1);
+   e->d_pl_nfnab.reserve(2 * (size_t)nc + 2);
    PLArgs a{e->nx, e->ny, e->nz, nc, noff, e->n_frame_types};
    k_pairlist<true><<<(nc + 3) / 4, 256, 0, e->stream>>>(
        a, e->d_off_pl.p, e->d_cell_start.p, e->d_cell_entries.p, e->d_entry_isite.p,
-       e->d_entry_num.p, e->d_entry_ftype.p, nullptr, e->d_pl_start.p, dn.p, dp.p, df.p);
-   if (nab) HIPCHK(hipMemcpyAsync(nab, dn.p, sizeof(int) * total, hipMemcpyDeviceToHost, e->stream));
-   if (pbctrans) HIPCHK(hipMemcpyAsync(pbctrans, dp.p, sizeof(int) * total, hipMemcpyDeviceToHost, e->stream));
-   if (nfnab) HIPCHK(hipMemcpyAsync(nfnab, df.p, sizeof(int) * 2 * nc, hipMemcpyDeviceToHost, e->stream));
+       e->d_entry_num.p, e->d_entry_ftype.p, nullptr, e->d_pl_start.p, e->d_pl_nab.p, e->d_pl_pbc.p,
+       e->d_pl_nfnab.p);
+}
+
+// ------------------------------------------------- radial distributions
+struct RdfArgs {
+   int ncells, nbins, max_id;
+   double rbin;
+   double reloc[81];
+};
+
+// rdf_inner (src/force.c:1010-1103) and rdf_accum (src/rdf.c:94-108): one
+// wavefront per cell walks the cell's sites in list order with the
+// reference's jmin/jmax rules, lanes take neighbours j, and every pair
+// closer than limit counts once in hist[block(id_i, id_j)][bin].  The
+// distance is formed in the reference's operation order without
+// contraction, so bins are exact.
+#pragma clang fp contract(off)
+static __global__ __launch_bounds__(256) void k_rdf(RdfArgs a, const int *__restrict__ c_start,
+                                                    const int *__restrict__ c_entries,
+                                                    const int *__restrict__ e_isite,
+                                                    const int *__restrict__ e_num,
+                                                    const int *__restrict__ e_ftype,
+                                                    const int *__restrict__ lstart, const int *__restrict__ nab,
+                                                    const int *__restrict__ pbct, const int *__restrict__ nfnab,
+                                                    const int *__restrict__ stype, const int *__restrict__ blk,
+                                                    const double *__restrict__ site, int nsp,
+                                                    int *__restrict__ hist) {
+   const int lane = threadIdx.x & 63;
+   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+   if (c >= a.ncells) return;
+   const int base = lstart[c], nnab = lstart[c + 1] - base, nf0 = nfnab[2 * c];
+   int jmin = 0;
+   for (int q = c_start[c]; q < c_start[c + 1]; q++) {
+      const int en = c_entries[q], ft = e_ftype[en];
+      int jmax;
+      if (ft) {
+         jmin = 0;
+         jmax = nf0;
+      } else {
+         jmax = nnab;
+      }
+      const int i0 = e_isite[en], i1 = i0 + e_num[en];
+      for (int i = i0; i < i1; i++) {
+         if (!ft) jmin++;
+         const double s0 = site[i], s1 = site[nsp + i], s2 = site[2 * nsp + i];
+         const int *brow = blk + stype[i] * a.max_id;
+         for (int j = jmin + lane; j < jmax; j += 64) {
+            const int sj = nab[base + j], t = pbct[base + j];
+            const double ax = site[sj] - s0 + a.reloc[3 * t];
+            const double ay = site[nsp + sj] - s1 + a.reloc[3 * t + 1];
+            const double az = site[2 * nsp + sj] - s2 + a.reloc[3 * t + 2];
+            const double rsq = ax * ax + ay * ay + az * az;
+            const int bin = (int)(a.rbin * sqrt(rsq));
+            if (bin < a.nbins) atomicAdd(hist + (size_t)brow[stype[sj]] * a.nbins + bin, 1);
+         }
+      }
+   }
+}
+
+// rdf_accum's float bins: n additions of 1/density to the float bin, each
+// rounded as (float)(bin + invrho) the way the reference's loop rounds them.
+static __global__ void k_rdf_float(int nb, double invrho, const int *__restrict__ hist,
+                                   float *__restrict__ rdf) {
+   int b = blockIdx.x * blockDim.x + threadIdx.x;
+   if (b >= nb) return;
+   float x = rdf[b];
+   for (int n = hist[b]; n > 0; n--) x = (float)((double)x + invrho);
+   rdf[b] = x;
+}
+#pragma clang fp contract(on)
+
+void mgpu::run_rdf(mgpu_engine *e, double limit, int nbins, int noff, float *rdf_host) {
+   const int nblk = (e->max_id - 1) * e->max_id / 2, nb = nblk * nbins;
+   std::vector<int> blk((size_t)e->max_id * e->max_id, 0);
+   for (int i = 1; i < e->max_id; i++)
+      for (int j = 1; j < e->max_id; j++) {
+         int lo = std::min(i, j), hi = std::max(i, j), k = 0;
+         for (int t = 1; t < lo; t++) k += e->max_id - t;
+         blk[(size_t)i * e->max_id + j] = k + (hi - lo);
+      }
+   e->d_rdf_blk.reserve(blk.size());
+   e->d_rdf_hist.reserve(nb);
+   e->d_rdf_bins.reserve(nb);
+   HIPCHK(hipMemcpyAsync(e->d_rdf_blk.p, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, e->stream));
+   HIPCHK(hipMemcpyAsync(e->d_rdf_bins.p, rdf_host, sizeof(float) * nb, hipMemcpyHostToDevice, e->stream));
+   zero_async(e, e->d_rdf_hist.p, sizeof(int) * nb);
+   RdfArgs a;
+   a.ncells = e->ncells; a.nbins = nbins; a.max_id = e->max_id;
+   a.rbin = nbins / limit;
+   int k = 0;
+   for (int ii = -1; ii <= 1; ii++)
+      for (int jj = -1; jj <= 1; jj++)
+         for (int kk = -1; kk <= 1; kk++, k++) {
+            a.reloc[3 * k] = e->h[0] * ii + e->h[1] * jj + e->h[2] * kk;
+            a.reloc[3 * k + 1] = e->h[3] * ii + e->h[4] * jj + e->h[5] * kk;
+            a.reloc[3 * k + 2] = e->h[6] * ii + e->h[7] * jj + e->h[8] * kk;
+         }
+   (void)noff;
+   {
+      ProfScope ps(e, "rdf");
+      k_rdf<<<(e->ncells + 3) / 4, 256, 0, e->stream>>>(
+          a, e->d_cell_start.p, e->d_cell_entries.p, e->d_entry_isite.p, e->d_entry_num.p, e->d_entry_ftype.p,
+          e->d_pl_start.p, e->d_pl_nab.p, e->d_pl_pbc.p, e->d_pl_nfnab.p, e->d_site_type.p, e->d_rdf_blk.p,
+          e->d_site.p, e->nsp, e->d_rdf_hist.p);
+      const double invrho = 1.0 / (e->nsites / host_det(e->h));
+      k_rdf_float<<<(nb + 255) / 256, 256, 0, e->stream>>>(nb, invrho, e->d_rdf_hist.p, e->d_rdf_bins.p);
+   }
+   HIPCHK(hipMemcpyAsync(rdf_host, e->d_rdf_bins.p, sizeof(float) * nb, hipMemcpyDeviceToHost, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
-   dn.free();
-   dp.free();
-   df.free();
 }

@@ -616,6 +616,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
    fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_pl_nab); fr(e->d_pl_pbc); fr(e->d_pl_nfnab); fr(e->d_rdf_blk); fr(e->d_rdf_hist); fr(e->d_rdf_bins);
    fr(e->d_mom); fr(e->d_amom); fr(e->d_scal); fr(e->d_trace); fr(e->d_spec_dyn);
    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
    if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -985,6 +986,35 @@ int mgpu_pairlist(mgpu_engine *e, const double h[9], const double *site, int ld,
       if (n < 0) return fail(e, -n, "cutoff radius exceeds the image-cell range (CUTOFF)");
       run_pairlist_count(e, strict, cutoff, off.data(), n, total, list_start);
       if (nab || pbctrans || nfnab) run_pairlist_fill(e, off.data(), n, *total, nab, pbctrans, nfnab);
+      return 0;
+   });
+}
+
+int mgpu_rdf(mgpu_engine *e, const double h[9], const double *site, int ld, const double *cm, double limit,
+             int nbins, float *rdf) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      if (!site || !rdf || ld < e->nsites || nbins <= 0 || !(limit > 0)) return fail(e, MGPU_E_ARG, "bad arguments");
+      if (e->molpbc && !cm) return fail(e, MGPU_E_ARG, "c_of_m required with molecular cutoff");
+      for (int t : e->site_type)
+         if (t < 1) return fail(e, MGPU_E_ARG, "site id 0 has no RDF block (rdf.c:80)");
+      HIPCHK(hipSetDevice(e->device));
+      set_geometry(e, h);
+      upload_sites(e, site, ld);
+      if (e->molpbc)
+         HIPCHK(hipMemcpyAsync(e->d_cm.p, cm, sizeof(double) * 3 * e->nmols, hipMemcpyHostToDevice, e->stream));
+      zero_async(e, e->d_iflags.p, sizeof(int) * 48);
+      run_binning(e, false);
+      sync_flags(e);
+      // strict neighbour list at limit over force_calc's cells, force.c:1306-1309
+      std::vector<int> off;
+      int n = neighbour_offsets(h, limit, e->nx, e->ny, e->nz, 1, off);
+      if (n < 0) return fail(e, -n, "RDF limit exceeds the image-cell range (CUTOFF)");
+      std::vector<long> starts((size_t)e->ncells + 1);
+      long total = 0;
+      run_pairlist_count(e, 1, limit, off.data(), n, &total, starts.data());
+      run_pairlist_fill_device(e, n, total);
+      run_rdf(e, limit, nbins, n, rdf);
       return 0;
    });
 }

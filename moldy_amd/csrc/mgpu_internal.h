@@ -132,6 +132,9 @@ struct mgpu_engine {
    mgpu::DBuf<float4> d_offD;                  // h (dx/nx,dy/ny,dz/nz) as float
    mgpu::DBuf<int4> d_off_pl;                  // offsets of an explicit pair-list call
    mgpu::DBuf<int> d_pl_count, d_pl_start;     // pair-list CSR counts / starts
+   mgpu::DBuf<int> d_pl_nab, d_pl_pbc, d_pl_nfnab;   // site neighbour lists (CSR)
+   mgpu::DBuf<int> d_rdf_blk, d_rdf_hist;      // RDF block table, pair counts
+   mgpu::DBuf<float> d_rdf_bins;               // RDF float bins
    double off_key[13] = {0};
    mgpu::DBuf<double> d_cell_part;             // per-cell energy/virial partials
 
@@ -211,6 +214,8 @@ void run_pairlist_count(mgpu_engine *e, int strict, double cutoff, const int *of
                         long *total, long *list_start_host);
 void run_pairlist_fill(mgpu_engine *e, const int *offs, int noff, long total, int *nab,
                        int *pbctrans, int *nfnab);
+void run_pairlist_fill_device(mgpu_engine *e, int noff, long total);
+void run_rdf(mgpu_engine *e, double limit, int nbins, int noff, float *rdf_host);
 void run_ewald_fp64(mgpu_engine *e);        // -> d_out[OUT_RECIP..]
 void run_ewald_mixed(mgpu_engine *e);       // -> d_out[OUT_RECIP..]
 // The device part of eval_forces after the two boundary calls: second

@@ -277,6 +277,21 @@ class ForceEngine:
                                       _p(nab, _ip), _p(pbc, _ip), _p(nf, _ip)))
         return dict(start=starts, nab=nab[:n], pbctrans=pbc[:n], nfnab=nf)
 
+    # ------------------------------------------ radial distribution functions
+    def rdf(self, h, xyz, limit, nbins, rdf=None, c_of_m=None):
+        """rdf_inner/rdf_accum on the GPU: float bins [(max_id-1)*max_id/2][nbins],
+        accumulated into ``rdf`` when given."""
+        s = self.sys
+        nblk = (s.max_id - 1) * s.max_id // 2
+        out = np.zeros((nblk, nbins), np.float32) if rdf is None else rdf
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == nblk * nbins
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        h = np.ascontiguousarray(h, np.float64)
+        cm = np.ascontiguousarray(s.c_of_m if c_of_m is None else c_of_m, np.float64)
+        self._chk(lib().mgpu_rdf(self.h_, _p(h), _p(xyz), s.nsites, _p(cm), C.c_double(limit),
+                                 int(nbins), out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     # --------------------------------------------------------- profiling
     def profile(self, on=True):
         lib().mgpu_profile_enable(self.h_, int(on))

@@ -22,6 +22,8 @@
 __attribute__((weak)) contr_mt control;
 __attribute__((weak)) int ithread = 0;
 __attribute__((weak)) int nthreads = 1;
+/* rdf.c's accumulator (src/rdf.c:59-63); absent outside Moldy */
+__attribute__((weak)) void *rdf_ptr(int *size);
 
 #define SEV_INFO 0
 #define SEV_WARNING 1
@@ -213,6 +215,18 @@ void force_calc(real **site, real **site_force, system_mt *system, spec_mt *spec
    for (k = 0; k < info.n_too_close && k < 16; k++)   /* src/force.c:944-946 */
       message(NULL, NULL, SEV_WARNING, "Sites %d and %d closer than %fA.", info.too_close_i[k],
               info.too_close_j[k], sqrt(0.25));
+   /* radial distribution functions on RDF steps, src/force.c:1302-1313 */
+   if (control.rdf_interval > 0 && control.istep >= control.begin_rdf &&
+       control.istep % control.rdf_interval == 0 && rdf_ptr) {
+      int size = 0;
+      float *base = (float *)rdf_ptr(&size);
+      if (base && size == control.nbins * system->max_id * (system->max_id - 1) / 2) {
+         rc = mgpu_rdf(eng, h, site[0], ld, (const double *)system->c_of_m, control.limit,
+                       control.nbins, base);
+         if (rc)
+            fatal_mgpu(rc);
+      }
+   }
 }
 
 /* src/ewald.c:280 */

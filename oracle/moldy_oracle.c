@@ -1697,3 +1697,113 @@ out:
    free(force);
    return rc;
 }
+
+/* ====================================================================== */
+/* Radial distribution functions: rdf_inner (src/force.c:1010-1103, run    */
+/* from force_calc at src/force.c:1302-1313) and rdf_accum (src/rdf.c:94-108) */
+/* ====================================================================== */
+
+int orc_rdf_block(int max_id, int idi, int idj)
+{
+   /* init_rdf, src/rdf.c:71-89: blocks for idi = 1.., idj = idi.. */
+   int a = idi < idj ? idi : idj, b = idi < idj ? idj : idi, k = 0, i;
+   for (i = 1; i < a; i++)
+      k += max_id - i;
+   return k + (b - a);
+}
+
+int orc_rdf(const orc_sys *s, const double h[9], const double *x, const double *y,
+            const double *z, const double *cm, double limit, int nbins, float *rdf)
+{
+   cells_t c;
+   int status = 0, rc, ic, isite, is, im, ispec;
+   int *id = NULL, *nab = NULL, *pbc = NULL, nfnab[2], ii, jj, kk, k;
+   double *rsq = NULL, reloc[27][3];
+   double density = s->nsites / orc_det(h), invrho = 1.0 / density, rbin = nbins / limit;
+   long maxlist = 0;
+
+   rc = cells_build(s, h, x, y, z, cm, 1, limit, &c, &status);
+   if (rc < 0) {
+      cells_free(&c);
+      return -rc;
+   }
+   id = (int *)malloc(sizeof(int) * (size_t)s->nsites);
+   if (!id) {
+      status = ORC_ERR_ALLOC;
+      goto done;
+   }
+   for (ispec = 0, isite = 0; ispec < s->nspecies; ispec++) {
+      int off = site_of_species(s, ispec);
+      for (im = 0; im < s->spec_nmols[ispec]; im++)
+         for (is = 0; is < s->spec_nsites[ispec]; is++, isite++) {
+            id[isite] = s->spec_site_id[off + is];
+            if (id[isite] < 1) {        /* rdf[0] is never allocated (rdf.c:80) */
+               status = ORC_ERR_MAP;
+               goto done;
+            }
+         }
+   }
+   for (ic = 0; ic < c.ncells; ic++) {
+      int ix = ic / (c.ny * c.nz), iy = ic / c.nz - c.ny * ix, iz = ic - c.nz * (iy + c.ny * ix);
+      long n = cell_site_list(&c, ix, iy, iz, NULL, NULL, NULL, 0);
+      if (n > maxlist)
+         maxlist = n;
+   }
+   maxlist += 1;
+   nab = (int *)malloc(sizeof(int) * (size_t)maxlist);
+   pbc = (int *)malloc(sizeof(int) * (size_t)maxlist);
+   rsq = (double *)malloc(sizeof(double) * (size_t)maxlist);
+   if (!nab || !pbc || !rsq) {
+      status = ORC_ERR_ALLOC;
+      goto done;
+   }
+   k = 0;
+   for (ii = -1; ii <= 1; ii++)
+      for (jj = -1; jj <= 1; jj++)
+         for (kk = -1; kk <= 1; kk++) {
+            reloc[k][0] = h[0] * ii + h[1] * jj + h[2] * kk;
+            reloc[k][1] = h[3] * ii + h[4] * jj + h[5] * kk;
+            reloc[k][2] = h[6] * ii + h[7] * jj + h[8] * kk;
+            k++;
+         }
+   for (ic = 0; ic < c.ncells; ic++) {
+      int ix, iy, iz, jmin, jmax, e, nnab, j;
+      if (c.cell_start[ic] == c.cell_start[ic + 1])
+         continue;
+      ix = ic / (c.ny * c.nz); iy = ic / c.nz - c.ny * ix; iz = ic - c.nz * (iy + c.ny * ix);
+      nnab = (int)cell_site_list(&c, ix, iy, iz, nab, pbc, nfnab, maxlist);
+      jmin = 0;
+      for (e = c.cell_start[ic]; e < c.cell_start[ic + 1]; e++) {
+         int en = c.cell_entries[e], ft = c.entry_ftype[en], lim;
+         if (ft) {
+            jmin = 0;
+            jmax = nfnab[0];
+         } else
+            jmax = nnab;
+         lim = c.entry_isite[en] + c.entry_num[en];
+         for (isite = c.entry_isite[en]; isite < lim; isite++) {
+            double s0 = x[isite], s1 = y[isite], s2 = z[isite];
+            if (!ft)
+               jmin++;
+            for (j = 0; j < jmax; j++) {
+               int t = pbc[j];
+               double ax = x[nab[j]] - s0 + reloc[t][0];
+               double ay = y[nab[j]] - s1 + reloc[t][1];
+               double az = z[nab[j]] - s2 + reloc[t][2];
+               rsq[j] = ax * ax + ay * ay + az * az;
+            }
+            for (j = jmin; j < jmax; j++) {
+               int bin = (int)(rbin * sqrt(rsq[j]));
+               if (bin < nbins) {
+                  float *b = rdf + (size_t)orc_rdf_block(s->max_id, id[isite], id[nab[j]]) * nbins;
+                  b[bin] += invrho;
+               }
+            }
+         }
+      }
+   }
+done:
+   free(id); free(nab); free(pbc); free(rsq);
+   cells_free(&c);
+   return status;
+}

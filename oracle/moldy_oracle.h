@@ -182,6 +182,14 @@ int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_st
                 int init_H_0, double pe[2], double stress[9], double rec[7]);
 int orc_symmetry_axis(const double *inertia);
 
+/* ---- radial distribution functions: rdf_inner (src/force.c:1010-1103) and
+ * rdf_accum (src/rdf.c:94-108).  Adds 1/density per site pair closer than
+ * limit (strict neighbour list at limit over force_calc's cells) into
+ * rdf[block(id_i, id_j)][nbins], float, blocks in init_rdf's order. ---- */
+int orc_rdf(const orc_sys *s, const double h[9], const double *x, const double *y,
+            const double *z, const double *c_of_m_frac, double limit, int nbins, float *rdf);
+int orc_rdf_block(int max_id, int idi, int idj);
+
 #ifdef __cplusplus
 }
 #endif

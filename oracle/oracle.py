@@ -259,6 +259,18 @@ class Oracle:
             return rc, pe, stress.reshape(3, 3), rec
         return step, st
 
+    def rdf(self, xyz, limit, nbins, rdf=None, c_of_m=None):
+        """rdf_inner/rdf_accum: float bins [(max_id-1)*max_id/2][nbins]."""
+        s = self.sys
+        nblk = (s.max_id - 1) * s.max_id // 2
+        out = np.zeros((nblk, nbins), np.float32) if rdf is None else rdf
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        cm = np.ascontiguousarray(s.c_of_m if c_of_m is None else c_of_m, np.float64)
+        rc = lib().orc_rdf(C.byref(self.cs), _p(self.h), _p(xyz[0]), _p(xyz[1]), _p(xyz[2]),
+                           _p(cm), C.c_double(limit), int(nbins),
+                           out.ctypes.data_as(C.POINTER(C.c_float)))
+        return rc, out
+
     def distant_const(self, iflag):
         return lib().orc_distant_const(C.byref(self.cs), C.c_double(self.sys.control.cutoff),
                                        iflag)

@@ -9,6 +9,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+# Moldy's rdf_ptr stand-in must be global before libmoldyforce.so loads
+RDF_STUB = os.path.join(ROOT, "tests", "abi_stub", "_build", "librdfstub.so")
+if os.path.exists(RDF_STUB):
+    import ctypes
+    ctypes.CDLL(RDF_STUB, mode=ctypes.RTLD_GLOBAL)
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
