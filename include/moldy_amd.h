@@ -159,6 +159,23 @@ int mgpu_pairlist(mgpu_engine *eng, const double h[9], const double *site, int l
 int mgpu_rdf(mgpu_engine *eng, const double h[9], const double *site, int ld,
              const double *c_of_m_frac, double limit, int nbins, float *rdf);
 
+/* ---- parameter adjoint (no counterpart in the reference; for force
+ * matching) ----
+ * theta = the n_potpar parameters of every site-id pair type a <= b
+ * (a, b = 0 .. max_id-1, row order), then the charge of every site id:
+ * ntheta = max_id (max_id + 1) / 2 * n_potpar + max_id (mgpu_theta_count).
+ * At the state of mgpu_upload_state, in FP64, for E = pe[0] + pe[1] and the
+ * site forces F of mgpu_eval_forces:
+ *   dE_dtheta[ntheta] = dE/dtheta (pair sums, intramolecular correction,
+ *                       distant terms, Ewald sum, self and sheet energies);
+ *   vjp[ntheta]       = sum_i v_i . dF_i/dtheta for the cotangent v[3][nsites];
+ *   jvp[3][nsites]    = sum_t dtheta_t dF/dtheta_t for the direction dtheta.
+ * Outputs may be NULL.  Charges cannot be differentiated with a framework or
+ * with the surface-dipole term (MGPU_E_CAPACITY). */
+int mgpu_theta_count(const mgpu_engine *eng, int *ntheta);
+int mgpu_adjoint(mgpu_engine *eng, const double *v, const double *dtheta, double *dE_dtheta,
+                 double *vjp, double *jvp);
+
 /* ---- timing helpers for measurement ---- */
 int mgpu_sync(mgpu_engine *eng);
 /* Average device time (ms) of the named kernel over the calls since

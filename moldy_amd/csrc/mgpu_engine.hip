@@ -591,6 +591,7 @@ int mgpu_create(mgpu_engine **eng, int device) {
       HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
       e->d_iflags.reserve(64);
       HIPCHK(hipMemset(e->d_iflags.p, 0, 64 * sizeof(int)));
+      e->d_out.reserve(OUT_SIZE);
       return 0;
    });
    if (rc) {
@@ -616,6 +617,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
    fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_adj); fr(e->d_adj_real);
    fr(e->d_pl_nab); fr(e->d_pl_pbc); fr(e->d_pl_nfnab); fr(e->d_rdf_blk); fr(e->d_rdf_hist); fr(e->d_rdf_bins);
    fr(e->d_mom); fr(e->d_amom); fr(e->d_scal); fr(e->d_trace); fr(e->d_spec_dyn);
    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
@@ -1015,6 +1017,26 @@ int mgpu_rdf(mgpu_engine *e, const double h[9], const double *site, int ld, cons
       run_pairlist_count(e, 1, limit, off.data(), n, &total, starts.data());
       run_pairlist_fill_device(e, n, total);
       run_rdf(e, limit, nbins, n, rdf);
+      return 0;
+   });
+}
+
+int mgpu_theta_count(const mgpu_engine *e, int *ntheta) {
+   if (!e || !ntheta) return MGPU_E_ARG;
+   if (!e->have_system) return MGPU_E_STATE;
+   *ntheta = adjoint_ntheta(e);
+   return 0;
+}
+
+int mgpu_adjoint(mgpu_engine *e, const double *v, const double *dtheta, double *dE_dtheta, double *vjp,
+                 double *jvp) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      if (e->n_full == 0) return fail(e, MGPU_E_STATE, "mgpu_upload_state not called");
+      if (vjp && !v) return fail(e, MGPU_E_ARG, "vjp needs the cotangent v");
+      if (jvp && !dtheta) return fail(e, MGPU_E_ARG, "jvp needs the direction dtheta");
+      HIPCHK(hipSetDevice(e->device));
+      run_adjoint(e, v, dtheta, dE_dtheta, vjp, jvp);
       return 0;
    });
 }

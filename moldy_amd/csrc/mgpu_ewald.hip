@@ -282,8 +282,9 @@ __global__ void k_add_parts(int ns, int nsp, int npart, const double *__restrict
 
 }  // namespace
 
-// Shared setup: trig tables (FP64) for the current sites.
-static void ewald_tables(mgpu_engine *e) {
+// Shared setup: trig tables (FP64) for the current sites, the z table
+// weighted by w (the site charges for the energy and forces).
+static void ewald_tables(mgpu_engine *e, const double *w) {
    size_t nsp = e->nsp;
    e->d_tx.reserve((e->hmax + 1) * nsp);
    e->d_ty.reserve((e->kmax + 1) * nsp);
@@ -295,17 +296,13 @@ static void ewald_tables(mgpu_engine *e) {
    t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = e->lmax; t.ns = e->nsites; t.nsp = e->nsp;
    ProfScope ps(e, "ewald_trig");
    k_trig<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(
-       t, e->d_site.p, e->d_chg_site.p, e->d_tx.p, e->d_ty.p, e->d_tz.p,
-       e->d_tz.p + (size_t)(e->lmax + 1) * nsp);
+       t, e->d_site.p, w, e->d_tx.p, e->d_ty.p, e->d_tz.p, e->d_tz.p + (size_t)(e->lmax + 1) * nsp);
 }
 
-void mgpu::run_ewald_fp64(mgpu_engine *e) {
-   ewald_tables(e);
-   const size_t nsp = e->nsp;
-   const double2 *tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * nsp;
-   // k groups: <= 256 k vectors, <= KROWS rows each, and row partitions of
-   // the force pass with about equal numbers of k vectors; both are host
-   // tables uploaded when the k set changes
+// k groups: <= 256 k vectors, <= KROWS rows each, and row partitions of
+// the force pass with about equal numbers of k vectors; both are host
+// tables uploaded when the k set changes
+static void ewald_plan(mgpu_engine *e) {
    if (e->ew_plan_version != e->geom_version) {
       std::vector<SkGroup> groups;
       int r = 0;
@@ -336,6 +333,13 @@ void mgpu::run_ewald_fp64(mgpu_engine *e) {
       e->ew_npart = npart;
       e->ew_plan_version = e->geom_version;
    }
+}
+
+void mgpu::run_ewald_fp64(mgpu_engine *e) {
+   ewald_plan(e);
+   ewald_tables(e, e->d_chg_site.p);
+   const size_t nsp = e->nsp;
+   const double2 *tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * nsp;
    int nchunks = (e->nsites + CHUNK - 1) / CHUNK;
    e->d_sk.reserve((size_t)nchunks * e->nk * 4 + 4);
    SkArgs s;
@@ -380,5 +384,253 @@ void mgpu::run_ewald_fp64(mgpu_engine *e) {
       k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
       k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p,
                                                                   e->d_force_site.p);
+   }
+}
+
+// ============================ charge adjoint ============================
+// With E = 1/2 sum_k coeff (C^2 + S^2), C + iS = sum_j q_j exp(i k.r_j) and
+// F_i = q_i sum_k coeff k (C sin_i - S cos_i) (ewald.c:539-579), the charge
+// derivatives by site id a use the id-restricted sums C_a + iS_a =
+// sum_{j in a} exp(i k.r_j):
+//   dE/dq_a       = sum_k coeff (C C_a + S S_a)
+//   dF_i/dq_a     = [id_i = a] phi_i + q_i sum_k coeff k (C_a sin_i - S_a cos_i)
+// with phi_i = sum_k coeff k (C sin_i - S cos_i), so that for a cotangent v
+//   sum_i v_i.dF_i/dq_a = sum_{i in a} v_i.phi_i + sum_k coeff (C_a T_S - S_a T_C),
+//   T_C + i T_S = sum_i q_i (v_i.k) exp(i k.r_i).
+namespace {
+
+__global__ void k_sf_sum(int nk, int nchunks, const double *__restrict__ part, double2 *__restrict__ out) {
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k >= nk) return;
+   double c = 0, s = 0;
+   for (int q = 0; q < nchunks; q++) {
+      const double *p = part + ((size_t)q * nk + k) * 4;
+      c += p[0];
+      s += p[1];
+   }
+   out[k] = make_double2(c, s);
+}
+
+struct KcArgs {
+   int nk;
+   double vol, r4a, c2;
+   const KRow *rows;
+   const int *krow;
+};
+
+// coeff and k of every k vector (ewald.c:506-531)
+__global__ void k_kcoeff(KcArgs a, double4 *__restrict__ out) {
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k >= a.nk) return;
+   KRow R = a.rows[a.krow[k]];
+   int l = R.l0 + (k - R.kstart);
+   double kx = R.kx, ky = R.ky, kz = R.kzt + l * a.c2;
+   double ksq = kx * kx + ky * ky + kz * kz;
+   out[k] = make_double4(kx, ky, kz, 2.0 / ((0.25 / PI_D) * a.vol) * exp(ksq * a.r4a) / ksq);
+}
+
+__global__ void k_weights_w(int nk, const double4 *__restrict__ kc, const double2 *__restrict__ cs,
+                            double *__restrict__ w) {
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k >= nk) return;
+   double c = kc[k].w;
+   w[4 * (size_t)k] = c * cs[k].x;
+   w[4 * (size_t)k + 1] = c * cs[k].y;
+   w[4 * (size_t)k + 2] = c * cs[k].x;
+   w[4 * (size_t)k + 3] = c * cs[k].y;
+}
+
+__global__ void k_site_weight(int ns, int mode, int id, const int *__restrict__ stype,
+                              const double *__restrict__ chg, const double *__restrict__ v, int nsp,
+                              double *__restrict__ w) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= nsp) return;
+   double x = 0.0;
+   if (i < ns) {
+      if (mode == 0) x = stype[i] == id ? 1.0 : 0.0;       // indicator of site id
+      else if (mode == 1) x = 1.0;                         // unit weights
+      else x = chg[i] * v[(size_t)(mode - 2) * nsp + i];  // q v_x, q v_y, q v_z
+   }
+   w[i] = x;
+}
+
+// per k: columns 2a (dE/dq_a) and 2a+1 (the k part of the cotangent product)
+__global__ __launch_bounds__(256) void k_charge_k(int nk, int nid, const double4 *__restrict__ kc,
+                                                  const double2 *__restrict__ csq,
+                                                  const double2 *__restrict__ csa,
+                                                  const double2 *__restrict__ csv, int have_v,
+                                                  double *__restrict__ part) {
+   __shared__ double ws[4][2 * MGPU_MAX_ID];
+   int k = blockIdx.x * 256 + threadIdx.x;
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   double4 K = k < nk ? kc[k] : make_double4(0, 0, 0, 0);
+   double2 cq = k < nk ? csq[k] : make_double2(0, 0);
+   double tc = 0, tsn = 0;
+   if (k < nk && have_v) {
+      double2 vx = csv[k], vy = csv[(size_t)nk + k], vz = csv[2 * (size_t)nk + k];
+      tc = K.x * vx.x + K.y * vy.x + K.z * vz.x;
+      tsn = K.x * vx.y + K.y * vy.y + K.z * vz.y;
+   }
+   for (int a = 0; a < nid; a++) {
+      double2 ca = k < nk ? csa[(size_t)a * nk + k] : make_double2(0, 0);
+      double de = K.w * (cq.x * ca.x + cq.y * ca.y);
+      double dv = K.w * (ca.x * tsn - ca.y * tc);
+      de = wave_sum(de);
+      dv = wave_sum(dv);
+      if (lane == 0) { ws[w][2 * a] = de; ws[w][2 * a + 1] = dv; }
+   }
+   __syncthreads();
+   if (threadIdx.x < 2 * nid)
+      part[(size_t)blockIdx.x * 2 * nid + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+// per site id a: sum_{i in a} v_i.phi_i
+__global__ __launch_bounds__(256) void k_vphi(int ns, int nsp, int nid, const int *__restrict__ stype,
+                                              const double *__restrict__ v, const double *__restrict__ phi,
+                                              double *__restrict__ part) {
+   __shared__ double ws[4][MGPU_MAX_ID];
+   int i = blockIdx.x * 256 + threadIdx.x;
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   double x = 0;
+   int t = -1;
+   if (i < ns) {
+      t = stype[i];
+      x = v[i] * phi[i] + v[nsp + i] * phi[nsp + i] + v[2 * (size_t)nsp + i] * phi[2 * (size_t)nsp + i];
+   }
+   for (int a = 0; a < nid; a++) {
+      double y = wave_sum(t == a ? x : 0.0);
+      if (lane == 0) ws[w][a] = y;
+   }
+   __syncthreads();
+   if (threadIdx.x < nid)
+      part[(size_t)blockIdx.x * nid + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+__global__ void k_colsum(int nrows, int ncols, const double *__restrict__ part, double *__restrict__ out) {
+   int c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= ncols) return;
+   double t = 0;
+   for (int r = 0; r < nrows; r++) t += part[(size_t)r * ncols + c];
+   out[c] = t;
+}
+
+// dS = sum_a dq_a (C_a, S_a)
+__global__ void k_dir_sf(int nk, int nid, const double *__restrict__ dq, const double2 *__restrict__ csa,
+                         double2 *__restrict__ out) {
+   int k = blockIdx.x * blockDim.x + threadIdx.x;
+   if (k >= nk) return;
+   double c = 0, s = 0;
+   for (int a = 0; a < nid; a++) {
+      double2 x = csa[(size_t)a * nk + k];
+      c += dq[a] * x.x;
+      s += dq[a] * x.y;
+   }
+   out[k] = make_double2(c, s);
+}
+
+// jvp_i += dq_{id_i} phi_i
+__global__ void k_jvp_phi(int ns, int nsp, const int *__restrict__ stype, const double *__restrict__ dq,
+                          const double *__restrict__ phi, double *__restrict__ out) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= ns) return;
+   double d = dq[stype[i]];
+   for (int c = 0; c < 3; c++) out[(size_t)c * nsp + i] += d * phi[(size_t)c * nsp + i];
+}
+
+}  // namespace
+
+// sum_i w_i exp(i k.r_i) for every k into cs[nk]
+static void weighted_sf(mgpu_engine *e, const double *w, double2 *cs) {
+   ewald_tables(e, w);
+   int nchunks = (e->nsites + CHUNK - 1) / CHUNK;
+   e->d_sk.reserve((size_t)nchunks * e->nk * 4 + 4);
+   SkArgs s;
+   s.nk = e->nk; s.ns = e->nsites; s.nsp = e->nsp; s.nsxf = e->nsites; s.lmax = e->lmax;
+   s.nchunks = nchunks; s.rows = e->d_rows.p; s.groups = (const SkGroup *)e->d_groups.p; s.krow = e->d_krow.p;
+   s.tx = e->d_tx.p; s.ty = e->d_ty.p; s.tz = e->d_tz.p; s.part = e->d_sk.p;
+   size_t shz = sizeof(double2) * (e->lmax + 1) * (TS + 1);
+   dim3 g(e->ew_ngroups, nchunks);
+   k_sfac<false><<<g, 256, shz, e->stream>>>(s);
+   k_sf_sum<<<(e->nk + 255) / 256, 256, 0, e->stream>>>(e->nk, nchunks, e->d_sk.p, cs);
+}
+
+// out[3][nsp] += w_i sum_k coeff k (C sin_i - S cos_i) with (C, S) = cs
+static void backproject(mgpu_engine *e, const double *w, const double4 *kc, const double2 *cs, double *out) {
+   e->d_wk.reserve((size_t)e->nk * 4 + 4);
+   k_weights_w<<<(e->nk + 255) / 256, 256, 0, e->stream>>>(e->nk, kc, cs, e->d_wk.p);
+   ewald_tables(e, w);
+   const int npart = e->ew_npart;
+   e->d_fpart.reserve((size_t)npart * 3 * e->nsp);
+   EfArgs ef;
+   ef.ns = e->nsites; ef.nsp = e->nsp; ef.nsxf = e->nsites; ef.nrows = e->nrows;
+   ef.c2 = 2 * PI_D * e->hinv[8]; ef.rows = e->d_rows.p; ef.rpart = e->d_rpart.p; ef.w = e->d_wk.p;
+   ef.tx = e->d_tx.p; ef.ty = e->d_ty.p; ef.tz = e->d_tz.p; ef.tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * e->nsp;
+   ef.fpart = e->d_fpart.p;
+   dim3 g((e->nsites + 255) / 256, npart);
+   k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
+   k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p, out);
+}
+
+void mgpu::ewald_charge_adjoint(mgpu_engine *e, const double *v, const double *dq_dev, double *dE, double *vjp,
+                                double *jvp) {
+   if (e->nsitesxf < e->nsites)
+      throw StatusError{MGPU_E_CAPACITY, "charge derivatives of the Ewald sum with a framework"};
+   ewald_plan(e);
+   const int nk = e->nk, nid = e->max_id, nsp = e->nsp, ns = e->nsites;
+   const int nb = (nsp + 255) / 256;
+   // scratch: coeff/k, C+iS for q, each id, q v_x..z, direction; phi; weights
+   size_t off = 0;
+   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+   const size_t oKc = take(sizeof(double4) * nk), oQ = take(sizeof(double2) * nk),
+                oA = take(sizeof(double2) * nk * (size_t)nid), oV = take(sizeof(double2) * nk * 3),
+                oD = take(sizeof(double2) * nk), oPhi = take(sizeof(double) * 3 * nsp),
+                oW = take(sizeof(double) * nsp), oP = take(sizeof(double) * (size_t)std::max(nk, nsp) / 256 * 2 * nid + 64 * nid);
+   e->d_adj.reserve(off);
+   uint8_t *B = e->d_adj.p;
+   double4 *kc = (double4 *)(B + oKc);
+   double2 *csq = (double2 *)(B + oQ), *csa = (double2 *)(B + oA), *csv = (double2 *)(B + oV),
+           *csd = (double2 *)(B + oD);
+   double *phi = (double *)(B + oPhi), *w = (double *)(B + oW), *part = (double *)(B + oP);
+   e->d_out.reserve(OUT_SIZE);
+   double *outc = e->d_out.p + OUT_TMP;
+   KcArgs ka{nk, host_det(e->h), -1.0 / (4.0 * e->alpha * e->alpha), 2 * PI_D * e->hinv[8], e->d_rows.p, e->d_krow.p};
+   k_kcoeff<<<(nk + 255) / 256, 256, 0, e->stream>>>(ka, kc);
+   weighted_sf(e, e->d_chg_site.p, csq);
+   for (int a = 0; a < nid; a++) {
+      k_site_weight<<<nb, 256, 0, e->stream>>>(ns, 0, a, e->d_site_type.p, e->d_chg_site.p, nullptr, nsp, w);
+      weighted_sf(e, w, csa + (size_t)a * nk);
+   }
+   if (v)
+      for (int c = 0; c < 3; c++) {
+         k_site_weight<<<nb, 256, 0, e->stream>>>(ns, 2 + c, 0, e->d_site_type.p, e->d_chg_site.p, v, nsp, w);
+         weighted_sf(e, w, csv + (size_t)c * nk);
+      }
+   // phi_i: unit site weights against the charges' structure factors
+   zero_async(e, phi, sizeof(double) * 3 * nsp);
+   k_site_weight<<<nb, 256, 0, e->stream>>>(ns, 1, 0, e->d_site_type.p, e->d_chg_site.p, nullptr, nsp, w);
+   backproject(e, w, kc, csq, phi);
+   // dE/dq and the k part of the cotangent product
+   const int nkb = (nk + 255) / 256;
+   k_charge_k<<<nkb, 256, 0, e->stream>>>(nk, nid, kc, csq, csa, csv, v != nullptr, part);
+   k_colsum<<<1, 64, 0, e->stream>>>(nkb, 2 * nid, part, outc);
+   std::vector<double> h2(2 * nid), hv(nid, 0.0);
+   HIPCHK(hipMemcpyAsync(h2.data(), outc, sizeof(double) * 2 * nid, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   if (v) {
+      k_vphi<<<nb, 256, 0, e->stream>>>(ns, nsp, nid, e->d_site_type.p, v, phi, part);
+      k_colsum<<<1, 64, 0, e->stream>>>(nb, nid, part, outc);
+      HIPCHK(hipMemcpyAsync(hv.data(), outc, sizeof(double) * nid, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+   }
+   for (int a = 0; a < nid; a++) {
+      if (dE) dE[a] = h2[2 * a];
+      if (vjp) vjp[a] = h2[2 * a + 1] + hv[a];
+   }
+   if (jvp && dq_dev) {
+      k_dir_sf<<<nkb, 256, 0, e->stream>>>(nk, nid, dq_dev, csa, csd);
+      backproject(e, e->d_chg_site.p, kc, csd, jvp);
+      k_jvp_phi<<<(ns + 255) / 256, 256, 0, e->stream>>>(ns, nsp, e->d_site_type.p, dq_dev, phi, jvp);
    }
 }

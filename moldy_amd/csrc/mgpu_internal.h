@@ -151,6 +151,8 @@ struct mgpu_engine {
    mgpu::DBuf<double> d_wk;                     // per-k weights
    mgpu::DBuf<int> d_rpart;                     // row partitions of the force pass
    mgpu::DBuf<double> d_fpart;                  // per-partition site forces
+   mgpu::DBuf<uint8_t> d_adj;                   // scratch of the parameter adjoint (k space)
+   mgpu::DBuf<uint8_t> d_adj_real;              // scratch of the parameter adjoint (real space)
    int sk_chunks = 0;
 
    // ---- mixed-precision ewald ----
@@ -223,6 +225,17 @@ void run_ewald_mixed(mgpu_engine *e);       // -> d_out[OUT_RECIP..]
 // correction and the assembly of pe, dipole and stress into d_out[OUT_RES..].
 void run_glue(mgpu_engine *e);
 void eval_forces_device(mgpu_engine *e, bool clear_flags);
+// Charge derivatives of the FP64 Ewald sum for the sites in d_site (no
+// frameworks): dE[max_id], the cotangent product vjp[max_id] for the site
+// force cotangent v (device [3][nsp], or null), and jvp (device [3][nsp],
+// accumulated) for the charge direction dq (device [max_id]).
+void ewald_charge_adjoint(mgpu_engine *e, const double *v, const double *dq_dev, double *dE, double *vjp,
+                          double *jvp);
+// Parameter adjoint (mgpu_adjoint.hip): theta = pair-type parameters then
+// site-id charges; host arrays v[3][nsites], dtheta/dE/vjp[ntheta],
+// jvp[3][nsites], each optional.
+int adjoint_ntheta(const mgpu_engine *e);
+void run_adjoint(mgpu_engine *e, const double *v, const double *dtheta, double *dE, double *vjp, double *jvp);
 // molecular dynamics (mgpu_dynamics.hip)
 void md_setup(mgpu_engine *e, const mgpu_dynamics *d);
 void md_set_scalars(mgpu_engine *e, const double thermo[3]);

@@ -81,6 +81,32 @@ def _p(a, t=_dp):
     return a.ctypes.data_as(t) if a is not None else None
 
 
+def get_theta(system):
+    """theta of the parameter adjoint: the n_potpar parameters of each
+    site-id pair a <= b (row order over 0..max_id-1), then the charges."""
+    m, npar = system.max_id, system.n_potpar
+    out = []
+    for a in range(m):
+        for b in range(a, m):
+            out.extend(system.potpar[a, b, :npar])
+    out.extend(system.site_charge)
+    return np.array(out, np.float64)
+
+
+def set_theta(system, theta):
+    """A copy of system with theta's parameters and charges."""
+    s = system.copy()
+    m, npar = s.max_id, s.n_potpar
+    q = 0
+    for a in range(m):
+        for b in range(a, m):
+            s.potpar[a, b, :npar] = theta[q:q + npar]
+            s.potpar[b, a, :npar] = theta[q:q + npar]
+            q += npar
+    s.site_charge[:] = theta[q:q + m]
+    return s
+
+
 class ForceEngine:
     """One engine (HIP stream + device buffers) bound to one system."""
 
@@ -89,6 +115,13 @@ class ForceEngine:
         h = C.c_void_p()
         self._chk(L.mgpu_create(C.byref(h), int(device)), h)
         self.h_ = h
+        self.rebind(system)
+        self.set_precision(precision)
+
+    def rebind(self, system):
+        """(Re)describe the system to the engine (mgpu_set_system); the state
+        must be uploaded again afterwards."""
+        L = lib()
         self.sys = system
         self.f = system.flat()
         f, c = self.f, system.control
@@ -101,7 +134,6 @@ class ForceEngine:
                              c.cutoff, c.subcell, c.alpha, c.k_cutoff, int(c.strict_cutoff),
                              int(c.molpbc), int(c.surface_dipole))
         self._chk(L.mgpu_set_system(self.h_, C.byref(self.cs)))
-        self.set_precision(precision)
         self.chg = system.site_charges()
 
     def _chk(self, rc, h=None):
@@ -276,6 +308,25 @@ class ForceEngine:
                                       C.c_double(cut), C.byref(total), _p(starts, _lp),
                                       _p(nab, _ip), _p(pbc, _ip), _p(nf, _ip)))
         return dict(start=starts, nab=nab[:n], pbctrans=pbc[:n], nfnab=nf)
+
+    # ------------------------------------------------- parameter adjoint
+    def theta_count(self):
+        n = C.c_int()
+        self._chk(lib().mgpu_theta_count(self.h_, C.byref(n)))
+        return n.value
+
+    def adjoint(self, v=None, dtheta=None, want_dE=True):
+        """dE/dtheta, sum_i v_i . dF_i/dtheta and sum_t dtheta_t dF/dtheta_t
+        at the uploaded state (theta layout: get_theta)."""
+        nt = self.theta_count()
+        ns = self.sys.nsites
+        dE = np.zeros(nt) if want_dE else None
+        vj = np.zeros(nt) if v is not None else None
+        jv = np.zeros((3, ns)) if dtheta is not None else None
+        vv = np.ascontiguousarray(v, np.float64) if v is not None else None
+        dt = np.ascontiguousarray(dtheta, np.float64) if dtheta is not None else None
+        self._chk(lib().mgpu_adjoint(self.h_, _p(vv), _p(dt), _p(dE), _p(vj), _p(jv)))
+        return dict(dE=dE, vjp=vj, jvp=jv)
 
     # ------------------------------------------ radial distribution functions
     def rdf(self, h, xyz, limit, nbins, rdf=None, c_of_m=None):
