@@ -119,6 +119,39 @@ def nve_run(s, start, precision, nsteps, dt=0.001, chunk=250):
                 ms_per_md_step=1e3 * (t1 - t0) / len(H), steps=len(H), dt_fs=dt * 1000)
 
 
+def replica_leg(nrep, steps, seed=9):
+    """64 rigid-water replicas of 256-2048 molecules (own box, alpha, r_c and
+    k_c by Moldy's rule, own temperature 250-400 K), mixed precision, stepped
+    together; schedule and launch shapes from the autotuner."""
+    from moldy_amd import engine as E, system
+    reps = system.water_replicas(nrep=nrep, seed=seed)
+    engines = []
+    for k, (s, T) in enumerate(reps):
+        e = E.ForceEngine(s, precision=E.MIXED)
+        e.set_dynamics(system.Dynamics(step=0.001))
+        e.upload_state()
+        e.upload_momenta(*s.thermal_momenta(T, seed=k))
+        engines.append(e)
+    b = E.Batch(engines)
+    b.steps(2, init_H_0=True)
+    sched, ms = b.autotune(trial_steps=4)
+    b.steps(2, init_H_0=True)                  # graph capture outside the timing
+    t0 = time.perf_counter()
+    b.steps(steps)
+    t1 = time.perf_counter()
+    dt = (t1 - t0) / steps
+    sites = [s.nsites for s, _ in reps]
+    b.close()
+    for e in engines:
+        e.close()
+    return dict(replicas=nrep, total_sites=int(sum(sites)), min_sites=int(min(sites)),
+                max_sites=int(max(sites)), schedule=E.SCHED_NAMES[sched],
+                autotune_ms_per_step={E.SCHED_NAMES[i]: float(ms[i]) for i in range(3)},
+                ms_per_step=1e3 * dt, aggregate_force_evals_per_s=nrep / dt,
+                per_replica_steps_per_s=1.0 / dt,
+                per_replica_ns_per_day=86400.0 * 1e-6 * 1.0 / dt, dt_fs=1.0)
+
+
 def cpu_baseline(s, budget_s=25.0):
     """Oracle (single core) on the same configuration: force_calc in full,
     ewald over a k-space subset, extrapolated linearly in the k count."""
@@ -155,6 +188,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="17k-site variant (smoke only)")
+    ap.add_argument("--replicas", type=int, default=64,
+                    help="replicas in the batched leg (0: skip)")
     ap.add_argument("--nve-steps", type=int, default=1000,
                     help="NVE steps per precision for the drift comparison (0: skip)")
     a = ap.parse_args()
@@ -233,6 +268,9 @@ def main():
         res["nve"] = nve
         res["md_ns_per_day"] = {k: 86400.0 * 1e-6 * nve[k]["dt_fs"] / (nve[k]["ms_per_md_step"] * 1e-3)
                                 for k in ("fp64", "mixed")}
+    if a.replicas > 0:
+        res["replica_batch"] = replica_leg(a.replicas, max(a.steps, 10))
+        print(f"[bench] replicas: {res['replica_batch']}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         cb = cpu_baseline(s)
         res["cpu_baseline"] = cb

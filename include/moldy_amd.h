@@ -176,6 +176,33 @@ int mgpu_theta_count(const mgpu_engine *eng, int *ntheta);
 int mgpu_adjoint(mgpu_engine *eng, const double *v, const double *dtheta, double *dE_dtheta,
                  double *vjp, double *jvp);
 
+/* ---- many replicas and the autotuner (no counterpart in the reference,
+ * which runs one system per process) ----
+ * A batch steps independent engines together (each prepared with
+ * mgpu_set_dynamics, mgpu_upload_state and mgpu_upload_momenta; uneven sizes
+ * and their own boxes and Ewald parameters allowed).  schedule:
+ *   MGPU_SCHED_GRAPH          one hipGraph with a branch per replica, one
+ *                             launch per step of the whole set;
+ *   MGPU_SCHED_STREAM_GRAPHS  each replica's step graph on its own stream;
+ *   MGPU_SCHED_STREAMS        eager launches on the replicas' streams;
+ *   -1                        the batch's current choice (autotuned).
+ * trace: [n_engines][nsteps][8] as mgpu_md_steps, or NULL.
+ * mgpu_batch_autotune times every engine's launch shapes (mgpu_autotune) and
+ * then each schedule over trial_steps MD steps, which advance the replicas;
+ * ms_per_step[3] receives the wall time per step of each schedule.
+ * mgpu_autotune picks e's launch shapes by timing its force evaluation:
+ * the S(k) MFMA workgroup target (mixed) or the FP64 back-projection row
+ * partitions; results are identical up to the order of partial sums. */
+enum { MGPU_SCHED_GRAPH = 0, MGPU_SCHED_STREAM_GRAPHS = 1, MGPU_SCHED_STREAMS = 2 };
+typedef struct mgpu_batch mgpu_batch;
+int  mgpu_batch_create(mgpu_batch **batch, int device);
+void mgpu_batch_destroy(mgpu_batch *batch);
+const char *mgpu_batch_last_error(const mgpu_batch *batch);
+int  mgpu_batch_add(mgpu_batch *batch, mgpu_engine *eng);
+int  mgpu_batch_steps(mgpu_batch *batch, int nsteps, int init_H_0, int schedule, double *trace);
+int  mgpu_batch_autotune(mgpu_batch *batch, int trial_steps, int *schedule, double ms_per_step[3]);
+int  mgpu_autotune(mgpu_engine *eng, int reps, int *sk_blocks, int *npart);
+
 /* ---- timing helpers for measurement ---- */
 int mgpu_sync(mgpu_engine *eng);
 /* Average device time (ms) of the named kernel over the calls since

@@ -25,7 +25,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INC, "-I" + CSRC]
 SOURCES = ["mgpu_engine.hip", "mgpu_cells.hip", "mgpu_realspace.hip", "mgpu_ewald.hip", "mgpu_dynamics.hip",
-           "mgpu_ewald_mixed.hip", "mgpu_glue.hip", "mgpu_adjoint.hip"]
+           "mgpu_ewald_mixed.hip", "mgpu_glue.hip", "mgpu_adjoint.hip",
+           "mgpu_batch.hip"]
 
 
 def _run(cmd):

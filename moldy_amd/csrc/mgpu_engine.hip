@@ -435,8 +435,10 @@ void download_forces(mgpu_engine *e, double *f, int ld) {
                             hipMemcpyDeviceToHost, e->stream));
 }
 
+}  // namespace
+
 // Bin/normalisation errors and too-close pairs from the flags block (host copy).
-void handle_flags(mgpu_engine *e, const int *fl) {
+void mgpu::handle_flags(mgpu_engine *e, const int *fl) {
    if (fl[0] & FLAG_BIN_RANGE) throw StatusError{MGPU_E_BIN_RANGE, "Co-ordinate out of range in BIN (fill_cells)"};
    if (fl[0] & FLAG_BIN_ROUND) throw StatusError{MGPU_E_BIN_ROUND, "Rounding problem in BIN (fill_cells)"};
    if (fl[0] & FLAG_QNORM)
@@ -449,6 +451,8 @@ void handle_flags(mgpu_engine *e, const int *fl) {
    fi.eintra = e->eintra;
    e->last_fc = fi;
 }
+
+namespace {
 
 void sync_flags(mgpu_engine *e) {
    int fl[41];
@@ -506,6 +510,38 @@ void recip_pass(mgpu_engine *e, double *pe, double st6[6], mgpu_ew_info *info) {
 }
 
 }  // namespace
+
+// A captured graph of one MD step (no H_0 evaluation) for e, rebuilt when the
+// geometry, the precision or any engine buffer changed since its capture.
+// The caller has run one eager step, which made every buffer.
+void mgpu::ensure_step_graph(mgpu_engine *e) {
+   if (e->graph_exec && e->graph_version == e->geom_version && e->graph_precision == e->precision &&
+       e->graph_epoch == g_alloc_epoch)
+      return;
+   if (e->graph_exec) HIPCHK(hipGraphExecDestroy(e->graph_exec));
+   if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
+   e->graph_exec = nullptr;
+   e->graph = nullptr;
+   unsigned long epoch = g_alloc_epoch;
+   e->capturing = true;
+   HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+   try {
+      md_step_launch(e, false);
+   } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(e->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      e->capturing = false;
+      throw;
+   }
+   HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
+   e->capturing = false;
+   if (epoch != g_alloc_epoch) throw StatusError{MGPU_E_STATE, "allocation during graph capture"};
+   HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+   e->graph_version = e->geom_version;
+   e->graph_precision = e->precision;
+   e->graph_epoch = g_alloc_epoch;
+}
 
 // eval_forces on the device (src/accel.c:398-617) from d_cm/d_quat: site
 // forces, molecular forces/torques and d_out[OUT_RES..]; no host
@@ -874,33 +910,7 @@ int mgpu_md_steps(mgpu_engine *e, int nsteps, int init_h0, int use_graph, double
       md_step_launch(e, init_h0 != 0);
       int done = 1;
       if (use_graph && nsteps > 1) {
-         if (!e->graph_exec || e->graph_version != e->geom_version || e->graph_precision != e->precision ||
-             e->graph_epoch != g_alloc_epoch) {
-            if (e->graph_exec) HIPCHK(hipGraphExecDestroy(e->graph_exec));
-            if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
-            e->graph_exec = nullptr;
-            e->graph = nullptr;
-            // the eager step above made every buffer; capture one more step
-            unsigned long epoch = g_alloc_epoch;
-            e->capturing = true;
-            HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            try {
-               md_step_launch(e, false);
-            } catch (...) {
-               hipGraph_t g = nullptr;
-               (void)hipStreamEndCapture(e->stream, &g);
-               if (g) (void)hipGraphDestroy(g);
-               e->capturing = false;
-               throw;
-            }
-            HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
-            e->capturing = false;
-            if (epoch != g_alloc_epoch) throw StatusError{MGPU_E_STATE, "allocation during graph capture"};
-            HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
-            e->graph_version = e->geom_version;
-            e->graph_precision = e->precision;
-            e->graph_epoch = g_alloc_epoch;
-         }
+         ensure_step_graph(e);
          for (; done < nsteps; done++) HIPCHK(hipGraphLaunch(e->graph_exec, e->stream));
       }
       for (; done < nsteps; done++) md_step_launch(e, false);

@@ -319,7 +319,7 @@ static void ewald_plan(mgpu_engine *e) {
       e->d_groups.reserve(groups.size() + 1);
       HIPCHK(hipMemcpyAsync(e->d_groups.p, groups.data(), sizeof(SkGroup) * groups.size(),
                             hipMemcpyHostToDevice, e->stream));
-      const int npart = std::max(1, std::min(16, e->nrows / 8));
+      const int npart = std::max(1, std::min(e->tune_npart, e->nrows / 8));
       std::vector<int> rp(npart + 1, e->nrows);
       rp[0] = 0;
       int p = 1;

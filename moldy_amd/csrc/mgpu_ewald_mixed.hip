@@ -564,7 +564,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    std::vector<int4> slices;
    std::vector<int> slice_fw;
    {
-      int want = std::max(1, std::min(64, (1024 + ntile - 1) / ntile));
+      int want = std::max(1, std::min(64, (e->tune_sk_blocks + ntile - 1) / ntile));
       auto add_range = [&](int lo, int hi, int fw) {
          if (hi <= lo) return;
          int n = hi - lo, per = std::max(KS, ((n + want - 1) / want + KS - 1) / KS * KS);

@@ -164,6 +164,9 @@ struct mgpu_engine {
    int geom_version = 0;
    int ew_plan_version = -1, mix_plan_version = -1;
    int ew_ngroups = 0, ew_npart = 0;
+   // launch shapes chosen by the autotuner (mgpu_autotune)
+   int tune_sk_blocks = 1024;                   // target S(k) MFMA workgroups
+   int tune_npart = 16;                         // FP64 back-projection row partitions
    void *mix_plan_ptr = nullptr;
    bool capturing = false;                      // inside hipStreamBeginCapture
 
@@ -242,6 +245,9 @@ void md_set_scalars(mgpu_engine *e, const double thermo[3]);
 void md_get_scalars(mgpu_engine *e, double thermo[3]);
 void md_reset_record(mgpu_engine *e, int cap);
 void md_step_launch(mgpu_engine *e, bool init_h0);
+void ensure_step_graph(mgpu_engine *e);
+void autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart);
+void handle_flags(mgpu_engine *e, const int *fl);   // throws on error flags
 int md_trace_width();
 // Fill with zeros by a kernel (safe inside a captured graph).
 void zero_async(mgpu_engine *e, void *p, size_t bytes);

@@ -107,6 +107,55 @@ def set_theta(system, theta):
     return s
 
 
+SCHED_GRAPH, SCHED_STREAM_GRAPHS, SCHED_STREAMS = 0, 1, 2
+SCHED_NAMES = ["graph", "stream_graphs", "streams"]
+
+
+class Batch:
+    """Replicas stepped together (mgpu_batch_*): engines prepared for MD."""
+
+    def __init__(self, engines, device=0):
+        L = lib()
+        L.mgpu_batch_last_error.restype = C.c_char_p
+        L.mgpu_batch_last_error.argtypes = [C.c_void_p]
+        L.mgpu_batch_destroy.argtypes = [C.c_void_p]
+        b = C.c_void_p()
+        rc = L.mgpu_batch_create(C.byref(b), int(device))
+        if rc:
+            raise EngineError(rc, "mgpu_batch_create")
+        self.b = b
+        self.engines = list(engines)
+        for e in self.engines:
+            self._chk(L.mgpu_batch_add(self.b, e.h_))
+
+    def _chk(self, rc):
+        if rc:
+            raise EngineError(rc, lib().mgpu_batch_last_error(self.b).decode())
+
+    def steps(self, nsteps, init_H_0=False, schedule=-1):
+        """Returns trace[n_replicas][nsteps][8] (mgpu_md_steps layout)."""
+        tr = np.zeros((len(self.engines), max(nsteps, 1), TRACE_W))
+        self._chk(lib().mgpu_batch_steps(self.b, int(nsteps), int(init_H_0), int(schedule), _p(tr)))
+        return tr[:, :nsteps]
+
+    def autotune(self, trial_steps=4):
+        s = C.c_int()
+        ms = np.zeros(3)
+        self._chk(lib().mgpu_batch_autotune(self.b, int(trial_steps), C.byref(s), _p(ms)))
+        return s.value, ms
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().mgpu_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class ForceEngine:
     """One engine (HIP stream + device buffers) bound to one system."""
 
@@ -308,6 +357,12 @@ class ForceEngine:
                                       C.c_double(cut), C.byref(total), _p(starts, _lp),
                                       _p(nab, _ip), _p(pbc, _ip), _p(nf, _ip)))
         return dict(start=starts, nab=nab[:n], pbctrans=pbc[:n], nfnab=nf)
+
+    def autotune(self, reps=3):
+        """Launch shapes by timing (mgpu_autotune): (sk_blocks, npart)."""
+        a, b = C.c_int(), C.c_int()
+        self._chk(lib().mgpu_autotune(self.h_, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # ------------------------------------------------- parameter adjoint
     def theta_count(self):

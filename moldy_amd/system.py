@@ -407,3 +407,23 @@ def silica(ncell=10, seed=7, disp=0.05, cutoff=10.0, alpha=0.30, accuracy=1e-6,
 def scaled_water(nmols, seed=1, **kw):
     """Small water boxes for tests (same model and density as config 2)."""
     return water_box(nmols, seed=seed, **kw)
+
+
+def water_replicas(nrep=64, seed=9, nmin=256, nmax=2048, tmin=250.0, tmax=400.0, rho=0.997,
+                   accuracy=1.013e-5):
+    """Independent rigid-water boxes of uneven size: molecule counts uniform in
+    [nmin, nmax] and temperatures uniform in [tmin, tmax] from the seed, each
+    box at density rho with its own alpha, cutoff and k-space cutoff from
+    Moldy's automatic rule (src/startup.c:713-772, ewald-accuracy default).
+    Returns a list of (system, temperature)."""
+    from moldy_amd import sysspec
+    rng = _rng(seed)
+    out = []
+    for r in range(nrep):
+        n = int(rng.integers(nmin, nmax + 1))
+        T = float(rng.uniform(tmin, tmax))
+        s = water_box(n, seed=seed * 1000 + r, rho=rho, name=f"replica{r}_{n}")
+        alpha, cutoff, kc = sysspec.init_cutoffs(0.0, 0.0, 0.0, s.h, s.nsites, accuracy)
+        s.control.alpha, s.control.cutoff, s.control.k_cutoff = alpha, cutoff, kc
+        out.append((s, T))
+    return out
