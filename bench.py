@@ -32,6 +32,12 @@ FLOPS_FORCE_PER_KI = 9.0
 # mk_forces 9), src/force.c:700-739, src/kernel.c:195-211.
 FLOPS_PER_PAIR = 59.0
 PEAK = {"fp64_vector": 78.6e12, "fp32_vector": 157.3e12, "fp16_mfma": 2.5e15}
+# HBM bytes per dispatch (read + write) from rocprofv3 FETCH_SIZE/WRITE_SIZE
+# passes on this workload, scaled by the counters' reading of a plain
+# 512 MiB copy (profiles/saltwater132872_eval_forces_hbm_traffic.md).
+MIB = float(1 << 20)
+TRAFFIC_BYTES = {"ewald_mfma_force": (257.98 + 3.04) * MIB, "ewald_mfma_sk": (686.40 + 82.57) * MIB,
+                 "ewald_sfac": (7008.12 + 52.26) * MIB, "ewald_force": (5336.46 + 48.66) * MIB}
 
 
 def build_workload(small=False):
@@ -188,6 +194,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="17k-site variant (smoke only)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device copy (the PMC byte-count reference)")
     ap.add_argument("--replicas", type=int, default=64,
                     help="replicas in the batched leg (0: skip)")
     ap.add_argument("--nve-steps", type=int, default=1000,
@@ -198,6 +206,15 @@ def main():
               "path is built for a single MI355X; see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_bytes_read=x.numel() * 4, copy_bytes_written=y.numel() * 4)))
+        return
     from moldy_amd import engine as E
     s = build_workload(a.small)
     print(f"[bench] workload {s.name}: {s.nsites} sites", file=sys.stderr, flush=True)
@@ -254,7 +271,7 @@ def main():
         ach = kind[1] / (prof[dom] * 1e-3)
         res["roofline"] = dict(kernel=dom, bound="mfma" if "mfma" in kind[0] else "valu",
                                achieved=ach / 1e12, peak=peak / 1e12, unit="TFLOP/s",
-                               frac=ach / peak, traffic=None,
+                               frac=ach / peak, traffic=TRAFFIC_BYTES.get(dom),
                                flops_per_launch=kind[1], avg_ms=prof[dom])
     res["kernel_ms"] = prof
     eng.close()
