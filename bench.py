@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="17k-site variant (smoke only)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="profiling runs: skip the FP64 leg and the error comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy (the PMC byte-count reference)")
     ap.add_argument("--replicas", type=int, default=64,
@@ -221,7 +223,14 @@ def main():
     eng = E.ForceEngine(s, precision=E.FP64)
     eng.upload_state()
 
-    # FP64 leg
+    if a.mixed_only:
+        eng.set_precision(E.MIXED)
+        eng.autotune()
+        tmx, profmx = time_gpu(eng, a.steps, a.warmup)
+        print(json.dumps(dict(ms_per_eval=1e3 * tmx, kernel_ms=profmx)))
+        return
+    # FP64 leg (launch shapes from the autotuner)
+    tuned = {"fp64": eng.autotune()}
     t64, prof64 = time_gpu(eng, a.steps, a.warmup)
     fc, ew = eng.last_info()
     pe64, _, st64 = eng.eval_forces_device()
@@ -234,6 +243,7 @@ def main():
     mixed = None
     if a.precision == "mixed":
         eng.set_precision(E.MIXED)
+        tuned["mixed"] = eng.autotune()
         tmx, profmx = time_gpu(eng, a.steps, a.warmup)
         pemx, _, stmx = eng.eval_forces_device()
         fmx, tqmx, _, sfmx = eng.download()
@@ -254,6 +264,10 @@ def main():
                            alpha_per_A=0.30, k_cutoff_per_A=round(s.control.k_cutoff, 4),
                            cells=[fc.nx, fc.ny, fc.nz], dt_fs=DT_FS,
                            precision=a.precision))
+    res["autotuned_shapes"] = {"fp64": dict(fp64_backprojection_parts=tuned["fp64"][1])}
+    if "mixed" in tuned:
+        res["autotuned_shapes"]["mixed"] = dict(sk_workgroups=tuned["mixed"][0],
+                                                backprojection_parts=tuned["mixed"][1])
     res["fp64_evals_per_s"] = 1.0 / t64
     res["fp64_ms_per_eval"] = 1e3 * t64
     if mixed:

@@ -191,8 +191,9 @@ int mgpu_adjoint(mgpu_engine *eng, const double *v, const double *dtheta, double
  * then each schedule over trial_steps MD steps, which advance the replicas;
  * ms_per_step[3] receives the wall time per step of each schedule.
  * mgpu_autotune picks e's launch shapes by timing its force evaluation:
- * the S(k) MFMA workgroup target (mixed) or the FP64 back-projection row
- * partitions; results are identical up to the order of partial sums. */
+ * the S(k) MFMA workgroup count and the MFMA back-projection row partitions
+ * (mixed; returned in sk_blocks and npart), or the FP64 back-projection row
+ * partitions (npart); results are identical up to the order of partial sums. */
 enum { MGPU_SCHED_GRAPH = 0, MGPU_SCHED_STREAM_GRAPHS = 1, MGPU_SCHED_STREAMS = 2 };
 typedef struct mgpu_batch mgpu_batch;
 int  mgpu_batch_create(mgpu_batch **batch, int device);

@@ -17,6 +17,7 @@
 #include "mgpu_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <map>
 #include <tuple>
@@ -157,34 +158,32 @@ void finish(mgpu_batch *b, int nsteps, double *trace) {
    }
 }
 
-// Process-wide memory of tuned shapes: (precision, nrows, lmax, nsites/4096).
-std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_tuned;
+// Process-wide memory of tuned shapes: (precision, nrows, lmax, nsites/4096)
+// -> (S(k) workgroups, MFMA back-projection partitions, FP64 partitions).
+std::map<std::tuple<int, int, int, int>, std::array<int, 3>> g_tuned;
 
 }  // namespace
 
 // Time the tunable launch shapes of e's force evaluation and keep the
-// fastest: the S(k) workgroup target of the MFMA Ewald (mixed precision) or
-// the row partitions of the FP64 back-projection.  Results are identical up
-// to the summation order of partial sums.
+// fastest: the S(k) workgroup count and the back-projection row partitions
+// of the MFMA Ewald (mixed precision), or the row partitions of the FP64
+// back-projection.  Results are identical up to the summation order of
+// partial sums.
 void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart) {
    if (e->alpha <= 1e-7) return;
    auto key = std::make_tuple(e->precision, e->nrows, e->lmax, e->nsites / 4096);
    auto it = g_tuned.find(key);
-   std::vector<int> cand = e->precision == MGPU_MIXED ? std::vector<int>{256, 512, 1024, 2048, 4096}
-                                                      : std::vector<int>{4, 8, 16, 32};
-   int best = e->precision == MGPU_MIXED ? e->tune_sk_blocks : e->tune_npart;
    if (it != g_tuned.end()) {
-      best = e->precision == MGPU_MIXED ? it->second.first : it->second.second;
+      e->tune_sk_blocks = it->second[0];
+      e->tune_force_parts = it->second[1];
+      e->tune_npart = it->second[2];
    } else {
       hipEvent_t t0, t1;
       HIPCHK(hipEventCreate(&t0));
       HIPCHK(hipEventCreate(&t1));
-      float best_ms = 1e30f;
       const bool prof = e->prof_on;
       e->prof_on = false;
-      for (int c : cand) {
-         if (e->precision == MGPU_MIXED) e->tune_sk_blocks = c;
-         else e->tune_npart = c;
+      auto time_eval = [&]() {
          e->geom_version++;   // re-plan with the new shape
          eval_forces_device(e, true);
          HIPCHK(hipEventRecord(t0, e->stream));
@@ -193,21 +192,35 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
          HIPCHK(hipEventSynchronize(t1));
          float ms = 0;
          HIPCHK(hipEventElapsedTime(&ms, t0, t1));
-         if (ms < best_ms) { best_ms = ms; best = c; }
+         return ms;
+      };
+      // coordinate descent over one shape at a time; workgroup targets in
+      // whole rounds of the resident slots (two per CU)
+      auto tune = [&](int &field, const std::vector<int> &cand) {
+         float best_ms = 1e30f;
+         int best = field;
+         for (int c : cand) {
+            field = c;
+            float ms = time_eval();
+            if (ms < best_ms) { best_ms = ms; best = c; }
+         }
+         field = best;
+      };
+      if (e->precision == MGPU_MIXED) {
+         const int slots = 2 * e->ncu;
+         tune(e->tune_sk_blocks, {slots, 2 * slots, 3 * slots, 4 * slots, 6 * slots});
+         tune(e->tune_force_parts, {1, 2, 4, 8, 16});
+      } else {
+         tune(e->tune_npart, {4, 8, 16, 32});
       }
       e->prof_on = prof;
       (void)hipEventDestroy(t0);
       (void)hipEventDestroy(t1);
-      auto &slot = g_tuned[key];
-      slot = {e->tune_sk_blocks, e->tune_npart};
-      if (e->precision == MGPU_MIXED) slot.first = best;
-      else slot.second = best;
+      g_tuned[key] = {e->tune_sk_blocks, e->tune_force_parts, e->tune_npart};
    }
-   if (e->precision == MGPU_MIXED) e->tune_sk_blocks = best;
-   else e->tune_npart = best;
    e->geom_version++;
-   if (sk_blocks) *sk_blocks = e->tune_sk_blocks;
-   if (npart) *npart = e->tune_npart;
+   if (sk_blocks) *sk_blocks = e->precision == MGPU_MIXED ? e->tune_sk_blocks : 0;
+   if (npart) *npart = e->precision == MGPU_MIXED ? e->tune_force_parts : e->tune_npart;
 }
 
 extern "C" {

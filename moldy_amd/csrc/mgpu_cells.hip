@@ -208,7 +208,7 @@ static __global__ void k_cell_finalize(FinArgs a, const int *__restrict__ c_star
                                        const int *__restrict__ stype, const int *__restrict__ smol,
                                        const int *__restrict__ sfw, double4 *__restrict__ cpos,
                                        float4 *__restrict__ cposf, int2 *__restrict__ cmeta,
-                                       int *__restrict__ corig) {
+                                       int *__restrict__ corig, int *__restrict__ ccell) {
    int c = blockIdx.x * blockDim.x + threadIdx.x;
    if (c >= a.ncells) return;
    int b = c_start[c], n = c_start[c + 1] - b;
@@ -237,6 +237,7 @@ static __global__ void k_cell_finalize(FinArgs a, const int *__restrict__ c_star
          cposf[pos] = make_float4((float)(x - o0), (float)(y - o1), (float)(z - o2), (float)q);
          cmeta[pos] = make_int2(stype[is] | (sfw[is] << 8), smol[is]);
          corig[pos] = is;
+         ccell[pos] = c;
       }
    }
 }
@@ -257,6 +258,7 @@ void mgpu::run_binning(mgpu_engine *e, bool) {
    e->d_cposf.reserve(e->nsites + 1);
    e->d_cmeta.reserve(e->nsites + 1);
    e->d_corig.reserve(e->nsites + 1);
+   e->d_ccell.reserve(e->nsites + 1);
    zero_async(e, e->d_cell_count.p, sizeof(int) * (nc + 1));
    zero_async(e, e->d_cell_nsites.p, sizeof(int) * (nc + 1));
    BinArgs a;
@@ -278,7 +280,7 @@ void mgpu::run_binning(mgpu_engine *e, bool) {
    k_cell_finalize<<<(nc + 255) / 256, 256, 0, e->stream>>>(
        f, e->d_cell_start.p, e->d_cell_entries.p, e->d_cell_site_start.p, e->d_entry_isite.p,
        e->d_entry_num.p, e->d_site.p, e->d_chg_site.p, e->d_site_type.p, e->d_site_mol.p,
-       e->d_site_fw.p, e->d_cpos.p, e->d_cposf.p, e->d_cmeta.p, e->d_corig.p);
+       e->d_site_fw.p, e->d_cpos.p, e->d_cposf.p, e->d_cmeta.p, e->d_corig.p, e->d_ccell.p);
 }
 
 // ------------------------------------------------------ site neighbour lists

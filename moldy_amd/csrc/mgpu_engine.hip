@@ -628,6 +628,10 @@ int mgpu_create(mgpu_engine **eng, int device) {
       e->d_iflags.reserve(64);
       HIPCHK(hipMemset(e->d_iflags.p, 0, 64 * sizeof(int)));
       e->d_out.reserve(OUT_SIZE);
+      hipDeviceProp_t prop;
+      HIPCHK(hipGetDeviceProperties(&prop, device));
+      e->ncu = std::max(1, prop.multiProcessorCount);
+      e->tune_sk_blocks = 4 * e->ncu;   // two rounds of two resident workgroups per CU
       return 0;
    });
    if (rc) {

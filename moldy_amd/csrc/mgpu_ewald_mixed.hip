@@ -343,6 +343,8 @@ struct FArgs {
    const half_t *Wt;
    const unsigned *maxbits;
    double *force;           // [3][nsp]
+   int nparts;              // row partitions (blockIdx.x % nparts)
+   float *fpart;            // [nparts][3][nsp] when nparts > 1
 };
 
 template <int NB>
@@ -354,8 +356,12 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    constexpr bool EARLY = NB <= 5;
    __shared__ __attribute__((aligned(16))) half_t sW[2][4][2][16][K2 + 8];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
-   const int base = a.s0 + blockIdx.x * BSITES;
-   const int ntile = a.nrows16 / 16;
+   // row partition p of the site block: consecutive workgroups take
+   // consecutive partitions, so each XCD's L2 holds one slice of the weights
+   const int part = blockIdx.x % a.nparts;
+   const int base = a.s0 + (blockIdx.x / a.nparts) * BSITES;
+   const int ntile_all = a.nrows16 / 16;
+   const int rt0 = part * ntile_all / a.nparts, ntile = (part + 1) * ntile_all / a.nparts;
    half8 wr[CPER];
    auto wload = [&](int rt) {
 #pragma unroll
@@ -377,7 +383,7 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
          }
       }
    };
-   wload(0);
+   wload(rt0);
    // Z operand [site][Zr | Zi over the padded l range], held for the whole kernel
    half8 zh[2][NB], zl[2][NB];
 #pragma unroll
@@ -421,8 +427,8 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    int cur_h = -1, cur_k = -1000000;
    wstore(0);
    __syncthreads();
-   for (int rt = 0; rt < ntile; rt++) {
-      const int buf = rt & 1;
+   for (int rt = rt0; rt < ntile; rt++) {
+      const int buf = (rt - rt0) & 1;
       if (EARLY && rt + 1 < ntile) wload(rt + 1);
       floatx4 acc[2][4];
 #pragma unroll
@@ -507,28 +513,33 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
          for (int j = 0; j < 4; j++) {
             const int i = base + 32 * wave + 16 * m + 4 * g + j;
             if (i < a.s1) {
-               a.force[i] += (double)f[m][j][0];
-               a.force[a.nsp + i] += (double)f[m][j][1];
-               a.force[2 * a.nsp + i] += (double)f[m][j][2];
+               if (a.nparts == 1) {
+                  a.force[i] += (double)f[m][j][0];
+                  a.force[a.nsp + i] += (double)f[m][j][1];
+                  a.force[2 * a.nsp + i] += (double)f[m][j][2];
+               } else {
+                  float *fp = a.fpart + (size_t)part * 3 * a.nsp;
+                  fp[i] = f[m][j][0];
+                  fp[a.nsp + i] = f[m][j][1];
+                  fp[2 * a.nsp + i] = f[m][j][2];
+               }
             }
          }
    }
 }
 
-__global__ void k_mix_sum(const double *__restrict__ in, int nrows, int ncols, double *__restrict__ out) {
-   __shared__ double ws[16];
-   int col = blockIdx.x;
-   double s = 0;
-   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
-   s = wave_sum(s);
-   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      double t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
-      out[col] = t;
+// Partition partials into the site forces, in partition order.
+__global__ void k_mix_fsum(int s0, int s1, int nsp, int nparts, const float *__restrict__ fpart,
+                           double *__restrict__ force) {
+   const int i = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= s1) return;
+   for (int c = 0; c < 3; c++) {
+      double t = 0.0;
+      for (int p = 0; p < nparts; p++) t += (double)fpart[((size_t)p * 3 + c) * nsp + i];
+      force[(size_t)c * nsp + i] += t;
    }
 }
+
 
 template <int NB> void launch_sk(dim3 g, hipStream_t s, const SkArgs &a) {
    k_mix_sk<NB><<<g, 256, 0, s>>>(a);
@@ -564,7 +575,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    std::vector<int4> slices;
    std::vector<int> slice_fw;
    {
-      int want = std::max(1, std::min(64, (e->tune_sk_blocks + ntile - 1) / ntile));
+      int want = std::max(1, std::min(64, e->tune_sk_blocks / ntile));
       auto add_range = [&](int lo, int hi, int fw) {
          if (hi <= lo) return;
          int n = hi - lo, per = std::max(KS, ((n + want - 1) / want + KS - 1) / KS * KS);
@@ -596,6 +607,8 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const size_t oMx = take(sizeof(unsigned) * 4);
    const int nred = (nrows * NCOL + 255) / 256;
    const size_t oBp = take(sizeof(double) * 8 * (size_t)nred);
+   const int nparts = std::max(1, std::min(e->tune_force_parts, nrows16 / 16));
+   const size_t oFp = take(nparts > 1 ? sizeof(float) * 3 * (size_t)nparts * nsp : 16);
    e->d_mix.reserve(off);
    uint8_t *B = e->d_mix.p;
    float2 *X = (float2 *)(B + oX), *Y = (float2 *)(B + oY);
@@ -610,6 +623,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    int *dsf = (int *)(B + oSf);
    unsigned *maxbits = (unsigned *)(B + oMx);
    double *bpart = (double *)(B + oBp);
+   float *fpart = (float *)(B + oFp);
    // row metadata and slices (host, from the k set): uploaded when the k set
    // or the scratch block changes
    if (e->mix_plan_version != e->geom_version || e->mix_plan_ptr != (void *)B) {
@@ -676,14 +690,19 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       f.rowhk = rowhk; f.rowk = rowk; f.X = X; f.Y = Y;
       f.Zhr = Zhr; f.Zlr = Zlr; f.Zhi = Zhi; f.Zli = Zli; f.maxbits = maxbits;
       f.force = e->d_force_site.p;
+      f.nparts = nparts;
+      f.fpart = fpart;
       const size_t shm = 0;
       auto run = [&](int s0, int s1, const half_t *wt) {
          if (s1 <= s0) return;
          f.s0 = s0; f.s1 = s1; f.Wt = wt;
-         dim3 g((s1 - s0 + BSITES - 1) / BSITES);
+         dim3 g((unsigned)(((s1 - s0 + BSITES - 1) / BSITES) * nparts));
 #define CALL_F(nb) launch_force<nb>(g, shm, e->stream, f)
          NB_SWITCH(NB, CALL_F)
 #undef CALL_F
+         if (nparts > 1)
+            k_mix_fsum<<<(s1 - s0 + 255) / 256, 256, 0, e->stream>>>(s0, s1, nsp, nparts, fpart,
+                                                                     e->d_force_site.p);
       };
       run(0, e->nsitesxf, Wt);
       run(e->nsitesxf, ns, Wtn);

@@ -10,7 +10,6 @@
 
 #define MGPU_NPOTP 8
 #define MGPU_MAX_ID 16          // site ids 0..15 (potential table in LDS)
-#define MGPU_IB 4               // i-sites register-blocked per wave
 #define MGPU_WAVE 64
 
 namespace mgpu {
@@ -122,6 +121,7 @@ struct mgpu_engine {
    mgpu::DBuf<int> d_entry_cell, d_entry_slot, d_cell_count, d_cell_start, d_cell_entries;
    mgpu::DBuf<int> d_cell_nsites, d_cell_site_start, d_scan_tmp;
    mgpu::DBuf<int> d_corig;                    // cell-ordered -> site index
+   mgpu::DBuf<int> d_ccell;                    // cell-ordered -> cell index
    mgpu::DBuf<double4> d_cpos;                 // cell-ordered x,y,z,q (FP64)
    mgpu::DBuf<float4> d_cposf;                 // cell-local x,y,z,q (FP32)
    mgpu::DBuf<int2> d_cmeta;                   // type | fw<<8, mol
@@ -165,8 +165,10 @@ struct mgpu_engine {
    int ew_plan_version = -1, mix_plan_version = -1;
    int ew_ngroups = 0, ew_npart = 0;
    // launch shapes chosen by the autotuner (mgpu_autotune)
-   int tune_sk_blocks = 1024;                   // target S(k) MFMA workgroups
+   int ncu = 256;                               // compute units of the device
+   int tune_sk_blocks = 1024;                   // most S(k) MFMA workgroups (2 per CU resident)
    int tune_npart = 16;                         // FP64 back-projection row partitions
+   int tune_force_parts = 8;                    // MFMA back-projection row partitions
    void *mix_plan_ptr = nullptr;
    bool capturing = false;                      // inside hipStreamBeginCapture
 

@@ -1,10 +1,10 @@
 // Real-space site-site forces (the work of force_inner + kernel + mk_forces,
 // src/force.c:784-1005, src/kernel.c:157-463).
 //
-// One wavefront owns one link cell and gathers, for each of its sites, the
-// interactions with every site in the full neighbour shell (the reference's
-// half-shell offsets plus their mirror images), so each unordered pair is
-// evaluated from both ends and no atomics are needed.  The pair set is the
+// Each site gathers its interactions with every site in the full neighbour
+// shell of its cell (the reference's half-shell offsets plus their mirror
+// images), so each unordered pair is evaluated from both ends and no
+// atomics are needed.  The pair set is the
 // reference's: pair (i,j) is present iff j lies in a cell offset by a
 // half-shell vector from i's cell (or the reverse), j != i, and not both are
 // framework sites (src/force.c:541-566, 904-912).  Energies and the virial
@@ -12,11 +12,10 @@
 // sum f r_a r_b, which equals the reference's sum r_i F_i + sum R_n g_n
 // (src/force.c:973-997) when site_force starts at zero.
 //
-// j sites are enumerated virtually: lanes take 64 neighbour cells at a time,
-// a wave prefix sum over their site counts numbers the candidate j sites, and
-// each lane locates its j by a binary search over that prefix (ds_bpermute).
-// Up to MGPU_IB i sites are register-blocked; forces on i are reduced with
-// wavefront shuffles at the end of each block.
+// Link cells hold about one site at the reference's subcell (r_c/5), so
+// the work is spread as one thread per (site, quarter of the shell): many
+// short, independent loops over L2-resident cell-ordered positions keep
+// enough waves in flight to hide the gather latency.
 #include "mgpu_internal.h"
 #include "mgpu_pair.h"
 
@@ -36,6 +35,8 @@ struct RSArgs {
    const float4 *cposf;
    const int2 *cmeta;
    const int *corig;
+   const int *ccell;
+   int nsites;
    const double *potpar;
    double *force;
    double *part;
@@ -58,6 +59,13 @@ template <> struct Pos<float> {
    }
 };
 
+// One thread per (site i, quarter of the full shell): the quarter takes
+// offsets o = q, q + NQ, ... (own cell in quarter 0) and walks the sites of
+// each neighbour cell.  The NQ threads of a site are adjacent lanes and
+// combine their forces with shuffles; energy and virial go to one partial
+// per wavefront.
+constexpr int NQ = 4;
+
 template <class R, int PT, bool COUL>
 __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
    __shared__ R spot[MGPU_MAX_ID * MGPU_MAX_ID * MGPU_NPOTP];
@@ -68,153 +76,96 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
    __syncthreads();
 
    const int lane = threadIdx.x & 63;
-   const int cell = blockIdx.x * 4 + (threadIdx.x >> 6);
-   if (cell >= a.ncells) return;
-   const int ix = cell / (a.ny * a.nz), iy = cell / a.nz - a.ny * ix,
-             iz = cell - a.nz * (iy + a.ny * ix);
-   const int cs = a.css[cell], ce = a.css[cell + 1];
-   double epot = 0, v00 = 0, v01 = 0, v02 = 0, v11 = 0, v12 = 0, v22 = 0;
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   const int p = t / NQ, qtr = t % NQ;
+   const bool have = p < a.nsites;
+   const int pi = have ? p : 0;
+   const int cell = a.ccell[pi];
+   const int ix = cell / (a.ny * a.nz), iy = cell / a.nz - a.ny * ix, iz = cell - a.nz * (iy + a.ny * ix);
+   R xi, yi, zi, qi;
+   Pos<R>::load(a, pi, xi, yi, zi, qi);
+   const int2 mi2 = a.cmeta[pi];
+   const int ti = mi2.x & 0xff, fwi = mi2.x >> 8, mi = mi2.y;
+   const R *prow = &spot[ti * a.max_id * MGPU_NPOTP];
    const R norm = (R)a.norm, alpha = (R)a.alpha;
-
-   for (int ib = cs; ib < ce; ib += MGPU_IB) {
-      const int ni = min(MGPU_IB, ce - ib);
-      R xi[MGPU_IB], yi[MGPU_IB], zi[MGPU_IB], qi[MGPU_IB];
-      int ti[MGPU_IB], mi[MGPU_IB], fwi[MGPU_IB];
-      R fxa[MGPU_IB], fya[MGPU_IB], fza[MGPU_IB];
-#pragma unroll
-      for (int k = 0; k < MGPU_IB; k++) {
-         int ii = ib + (k < ni ? k : 0);
-         Pos<R>::load(a, ii, xi[k], yi[k], zi[k], qi[k]);
-         int2 m = a.cmeta[ii];
-         ti[k] = m.x & 0xff;
-         fwi[k] = m.x >> 8;
-         mi[k] = m.y;
-         fxa[k] = fya[k] = fza[k] = 0;
+   // per-cell partials in R, the running sums in FP64
+   double gx = 0, gy = 0, gz = 0, v[7] = {0, 0, 0, 0, 0, 0, 0};
+   for (int o = qtr; have && o < a.nfull; o += NQ) {
+      R fx = 0, fy = 0, fz = 0, ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
+      const int4 d = a.off[o];
+      int ex = ix + d.x, ey = iy + d.y, ez = iz + d.z;
+      int sx = ex < 0 ? -1 : (ex >= a.nx ? 1 : 0);
+      int sy = ey < 0 ? -1 : (ey >= a.ny ? 1 : 0);
+      int sz = ez < 0 ? -1 : (ez >= a.nz ? 1 : 0);
+      const int jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
+      R shx, shy, shz;
+      if (sizeof(R) == 8) {
+         const int code = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
+         shx = (R)sreloc[3 * code]; shy = (R)sreloc[3 * code + 1]; shz = (R)sreloc[3 * code + 2];
+      } else {
+         const float4 D = a.offD[o];
+         shx = D.x; shy = D.y; shz = D.z;
       }
-      for (int o0 = 0; o0 < a.nfull; o0 += 64) {
-         const int o = o0 + lane;
-         int cnt = 0, jst = 0, code = 13;
-         if (o < a.nfull) {
-            int4 d = a.off[o];
-            int ex = ix + d.x, ey = iy + d.y, ez = iz + d.z;
-            int sx = ex < 0 ? -1 : (ex >= a.nx ? 1 : 0);
-            int sy = ey < 0 ? -1 : (ey >= a.ny ? 1 : 0);
-            int sz = ez < 0 ? -1 : (ez >= a.nz ? 1 : 0);
-            int jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
-            jst = a.css[jc];
-            cnt = a.css[jc + 1] - jst;
-            code = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
-         }
-         const int inc = wave_incl_scan(cnt, lane);
-         const int tot = __shfl(inc, 63, 64);
-         R ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
-         for (int t0 = 0; t0 < tot; t0 += 64) {
-            const int kq = t0 + lane;
-            const bool active = kq < tot;
-            int lo = 0, hi = 63;
-#pragma unroll
-            for (int s = 0; s < 6; s++) {
-               int mid = (lo + hi) >> 1;
-               int vm = __shfl(inc, mid, 64);
-               if (vm > kq) hi = mid; else lo = mid + 1;
-            }
-            const int L = lo;
-            const int incL = __shfl(inc, L, 64), cntL = __shfl(cnt, L, 64);
-            const int jstL = __shfl(jst, L, 64), codeL = __shfl(code, L, 64);
-            int j = jstL + (kq - (incL - cntL));
-            if (!active) j = cs;
-            const int oi = min(o0 + L, a.nfull - 1);   // inactive lanes: any valid offset
-            R xj, yj, zj, qj;
-            Pos<R>::load(a, j, xj, yj, zj, qj);
-            const int2 mj = a.cmeta[j];
-            const int tj = mj.x & 0xff, fwj = mj.x >> 8;
-            R shx, shy, shz;
-            if (sizeof(R) == 8) {
-               shx = (R)sreloc[3 * codeL]; shy = (R)sreloc[3 * codeL + 1]; shz = (R)sreloc[3 * codeL + 2];
-            } else {
-               float4 D = a.offD[oi];
-               shx = D.x; shy = D.y; shz = D.z;
-            }
-#pragma unroll
-            for (int k = 0; k < MGPU_IB; k++) {
-               if (k >= ni) break;
-               const bool skip = !active || (oi == 0 && j == ib + k) || (fwi[k] & fwj);
-               R rx = xj - xi[k] + shx, ry = yj - yi[k] + shy, rz = zj - zi[k] + shz;
-               R rsq = rx * rx + ry * ry + rz * rz;
-               if (!skip && rsq < R(0.25) && mi[k] != mj.y) {
-                  int oiorig = a.corig[ib + k], ojorig = a.corig[j];
-                  if (oiorig < ojorig) {
-                     int slot = atomicAdd(a.tooclose, 1);
-                     if (slot < 16) {
-                        a.tooclose[1 + 2 * slot] = oiorig;
-                        a.tooclose[2 + 2 * slot] = ojorig;
-                     }
-                  }
+      const int j1 = a.css[jc + 1];
+      for (int j = a.css[jc]; j < j1; j++) {
+         if (o == 0 && j == p) continue;
+         const int2 mj = a.cmeta[j];
+         const int tj = mj.x & 0xff, fwj = mj.x >> 8;
+         if (fwi & fwj) continue;
+         R xj, yj, zj, qj;
+         Pos<R>::load(a, j, xj, yj, zj, qj);
+         R rx = xj - xi + shx, ry = yj - yi + shy, rz = zj - zi + shz;
+         R rsq = rx * rx + ry * ry + rz * rz;
+         if (rsq < R(0.25) && mi != mj.y) {
+            int io = a.corig[p], jo = a.corig[j];
+            if (io < jo) {
+               int slot = atomicAdd(a.tooclose, 1);
+               if (slot < 16) {
+                  a.tooclose[1 + 2 * slot] = io;
+                  a.tooclose[2 + 2 * slot] = jo;
                }
-               if (a.strict_sub && rsq > (R)a.cutsq) rsq = (R)a.cut100sq;
-               if (skip) rsq = R(1);
-               R u, f;
-               pair_term<R, PT, COUL>(rsq, qi[k] * qj, norm, alpha,
-                                      &spot[(ti[k] * a.max_id + tj) * MGPU_NPOTP], u, f);
-               if (skip) { u = 0; f = 0; }
-               // The minimum-image intramolecular pairs of rigid molecules sum
-               // to eintra, which the FP64 path subtracts again; the FP32 path
-               // leaves both out so their large cancelling energies never
-               // meet FP32 rounding.  Forces are kept.
-               if (sizeof(R) == 4 && a.skip_intra_u && mi[k] == mj.y && rsq < (R)a.half_box_sq) u = 0;
-               fxa[k] -= f * rx;
-               fya[k] -= f * ry;
-               fza[k] -= f * rz;
-               ep += u;
-               w00 += f * rx * rx; w01 += f * rx * ry; w02 += f * rx * rz;
-               w11 += f * ry * ry; w12 += f * ry * rz; w22 += f * rz * rz;
             }
          }
-         epot += (double)ep;
-         v00 += (double)w00; v01 += (double)w01; v02 += (double)w02;
-         v11 += (double)w11; v12 += (double)w12; v22 += (double)w22;
+         if (a.strict_sub && rsq > (R)a.cutsq) rsq = (R)a.cut100sq;
+         R u, f;
+         pair_term<R, PT, COUL>(rsq, qi * qj, norm, alpha, prow + tj * MGPU_NPOTP, u, f);
+         // The minimum-image intramolecular pairs of rigid molecules sum to
+         // eintra, which the FP64 path subtracts again; the FP32 path leaves
+         // both out so their large cancelling energies never meet FP32
+         // rounding.  Forces are kept.
+         if (sizeof(R) == 4 && a.skip_intra_u && mi == mj.y && rsq < (R)a.half_box_sq) u = 0;
+         fx -= f * rx;
+         fy -= f * ry;
+         fz -= f * rz;
+         ep += u;
+         w00 += f * rx * rx; w01 += f * rx * ry; w02 += f * rx * rz;
+         w11 += f * ry * ry; w12 += f * ry * rz; w22 += f * rz * rz;
       }
+      gx += (double)fx; gy += (double)fy; gz += (double)fz;
+      v[0] += (double)ep;
+      v[1] += (double)w00; v[2] += (double)w01; v[3] += (double)w02;
+      v[4] += (double)w11; v[5] += (double)w12; v[6] += (double)w22;
+   }
+   // forces: the NQ quarters of a site sit in adjacent lanes
 #pragma unroll
-      for (int k = 0; k < MGPU_IB; k++) {
-         if (k >= ni) break;
-         double gx = wave_sum((double)fxa[k]);
-         double gy = wave_sum((double)fya[k]);
-         double gz = wave_sum((double)fza[k]);
-         if (lane == 0) {
-            int io = a.corig[ib + k];
-            a.force[io] += gx;
-            a.force[a.nsp + io] += gy;
-            a.force[2 * a.nsp + io] += gz;
-         }
-      }
+   for (int m = 1; m < NQ; m <<= 1) {
+      gx += __shfl_xor(gx, m, 64);
+      gy += __shfl_xor(gy, m, 64);
+      gz += __shfl_xor(gz, m, 64);
    }
-   epot = wave_sum(epot);
-   v00 = wave_sum(v00); v01 = wave_sum(v01); v02 = wave_sum(v02);
-   v11 = wave_sum(v11); v12 = wave_sum(v12); v22 = wave_sum(v22);
+   if (have && qtr == 0) {
+      const int io = a.corig[p];
+      a.force[io] += gx;
+      a.force[a.nsp + io] += gy;
+      a.force[2 * a.nsp + io] += gz;
+   }
+#pragma unroll
+   for (int q = 0; q < 7; q++) v[q] = wave_sum(v[q]);
    if (lane == 0) {
-      double *p = a.part + 8 * (size_t)cell;
-      p[0] = 0.5 * epot;
-      p[1] = 0.5 * v00; p[2] = 0.5 * v01; p[3] = 0.5 * v02;
-      p[4] = 0.5 * v11; p[5] = 0.5 * v12; p[6] = 0.5 * v22;
-      p[7] = 0;
-   }
-}
-
-// Deterministic reduction of per-cell partials: out[c] = sum over rows.
-__global__ void k_reduce_rows(const double *__restrict__ in, int nrows, int ncols,
-                              double *__restrict__ out) {
-   // one block per column
-   __shared__ double ws[16];
-   int col = blockIdx.x;
-   double s = 0;
-   for (int r = threadIdx.x; r < nrows; r += blockDim.x) s += in[(size_t)r * ncols + col];
-   s = wave_sum(s);
-   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      double t = 0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += ws[w];
-      out[col] = t;
+      double *pp = a.part + 8 * (size_t)(t >> 6);
+      pp[0] = 0.5 * v[0];
+      for (int q = 1; q < 7; q++) pp[q] = 0.5 * v[q];
+      pp[7] = 0;
    }
 }
 
@@ -235,7 +186,8 @@ void launch_pt(int pt, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
 
 void mgpu::run_realspace(mgpu_engine *e) {
    int nc = e->ncells;
-   e->d_cell_part.reserve((size_t)nc * 8 + 8);
+   const int nthreads = e->nsites * NQ, nblk = (nthreads + 255) / 256, nwaves = nblk * 4;
+   e->d_cell_part.reserve((size_t)nwaves * 8 + 8);
    RSArgs a;
    a.ncells = nc; a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.nfull = e->n_full;
    a.max_id = e->max_id; a.nsp = e->nsp;
@@ -262,11 +214,13 @@ void mgpu::run_realspace(mgpu_engine *e) {
    a.cposf = e->d_cposf.p;
    a.cmeta = e->d_cmeta.p;
    a.corig = e->d_corig.p;
+   a.ccell = e->d_ccell.p;
+   a.nsites = e->nsites;
    a.potpar = e->d_potpar.p;
    a.force = e->d_force_site.p;
    a.part = e->d_cell_part.p;
    a.tooclose = e->d_iflags.p + 1;
-   dim3 g((nc + 3) / 4), b(256);
+   dim3 g(nblk), b(256);
    bool coul = e->alpha > 0.0;
    {
       ProfScope ps(e, "realspace");
@@ -279,5 +233,5 @@ void mgpu::run_realspace(mgpu_engine *e) {
       }
    }
    e->d_out.reserve(OUT_SIZE);
-   reduce_cols(e, e->d_cell_part.p, nc, 8, e->d_out.p + OUT_REAL);
+   reduce_cols(e, e->d_cell_part.p, nwaves, 8, e->d_out.p + OUT_REAL);
 }
