@@ -74,6 +74,30 @@ def time_gpu(eng, steps, warmup):
     return (t1 - t0) / steps, prof
 
 
+def pairlist_leg(eng, s, reps=3):
+    """Link-cell binning and the reference's site neighbour lists (nab,
+    pbctrans) for every cell, built on the GPU at the force cutoff; device
+    time from HIP events on the engine stream.  Algorithmic bytes: binning
+    32 B per site; lists 8 B per entry plus 8 B per cell."""
+    eng.upload_state()
+    eng.eval_forces_device()
+    _, _, xyz, _ = eng.download()
+    eng.profile(True)
+    for _ in range(reps):
+        out = eng.pairlist(s.h, xyz)
+    t_bin, _ = eng.profile_get("binning")
+    t_cnt, _ = eng.profile_get("pairlist_count")
+    t_fill, _ = eng.profile_get("pairlist_fill")
+    eng.profile(False)
+    total = int(out["start"][-1])
+    ncells = len(out["start"]) - 1
+    list_bytes = 8.0 * total + 8.0 * ncells
+    return dict(entries=total, cells=ncells, binning_ms=t_bin, count_ms=t_cnt, fill_ms=t_fill,
+                binning_GBps=32.0 * s.nsites / (t_bin * 1e-3) / 1e9,
+                list_fill_GBps=list_bytes / (t_fill * 1e-3) / 1e9,
+                list_build_GBps=list_bytes / ((t_cnt + t_fill) * 1e-3) / 1e9)
+
+
 def equilibrate(s, T=300.0, nblocks=8, nper=150, dt=0.001, precision=1, seed=7):
     """Relax the lattice start: short NVE blocks with the momenta rescaled to
     T after each (translational and rotational parts separately)."""
@@ -288,6 +312,8 @@ def main():
                                frac=ach / peak, traffic=TRAFFIC_BYTES.get(dom),
                                flops_per_launch=kind[1], avg_ms=prof[dom])
     res["kernel_ms"] = prof
+    res["pairlist"] = pairlist_leg(eng, s)
+    print(f"[bench] pairlist: {res['pairlist']}", file=sys.stderr, flush=True)
     eng.close()
     if a.nve_steps > 0:
         start = equilibrate(s)

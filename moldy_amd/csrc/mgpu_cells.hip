@@ -290,14 +290,16 @@ struct PLArgs {
 
 // One wavefront per reference cell.  Lanes take offsets; each lane counts
 // the sites its neighbour cell contributes for the current frame type, a
-// wave prefix sum places them, and (fill mode) the lane writes them.
+// wave prefix sum places them, and (fill mode) the lane writes them.  The
+// cell-ordered arrays hold each cell's sites contiguously in the reference's
+// list order (entries in linked-list order, a molecule's sites in order), so
+// a neighbour cell's contribution is its run of corig, filtered by the frame
+// flag when a framework is present.
 template <bool FILL>
 static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *__restrict__ off,
-                                                         const int *__restrict__ c_start,
-                                                         const int *__restrict__ c_entries,
-                                                         const int *__restrict__ e_isite,
-                                                         const int *__restrict__ e_num,
-                                                         const int *__restrict__ e_ftype,
+                                                         const int *__restrict__ css,
+                                                         const int *__restrict__ corig,
+                                                         const int2 *__restrict__ cmeta,
                                                          int *__restrict__ count,
                                                          const int *__restrict__ lstart,
                                                          int *__restrict__ nab, int *__restrict__ pbct,
@@ -310,31 +312,31 @@ static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *_
    int nn = 0;
    for (int ft = 0; ft < a.nftypes; ft++) {
       for (int o0 = (ft == 0 ? 0 : 1); o0 < a.noff; o0 += 64) {
-         int o = o0 + lane, cnt = 0, jc = 0, code = 0;
+         int o = o0 + lane, cnt = 0, j0 = 0, j1 = 0, code = 0;
          if (o < a.noff) {
             int4 d = off[o];
             int ex = ix + d.x, ey = iy + d.y, ez = iz + d.z;
             int sx = ex < 0 ? -1 : (ex >= a.nx ? 1 : 0);
             int sy = ey < 0 ? -1 : (ey >= a.ny ? 1 : 0);
             int sz = ez < 0 ? -1 : (ez >= a.nz ? 1 : 0);
-            jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
+            int jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
             code = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
-            for (int q = c_start[jc]; q < c_start[jc + 1]; q++) {
-               int en = c_entries[q];
-               if (e_ftype[en] == ft) cnt += e_num[en];
+            j0 = css[jc];
+            j1 = css[jc + 1];
+            if (a.nftypes == 1) {
+               cnt = j1 - j0;
+            } else {
+               for (int j = j0; j < j1; j++) cnt += (cmeta[j].x >> 8) == ft;
             }
          }
          int inc = wave_incl_scan(cnt, lane);
          if (FILL && cnt) {
             int p = base + nn + inc - cnt;
-            for (int q = c_start[jc]; q < c_start[jc + 1]; q++) {
-               int en = c_entries[q];
-               if (e_ftype[en] != ft) continue;
-               int s0 = e_isite[en], m = e_num[en];
-               for (int k = 0; k < m; k++, p++) {
-                  nab[p] = s0 + k;
-                  pbct[p] = code;
-               }
+            for (int j = j0; j < j1; j++) {
+               if (a.nftypes != 1 && (cmeta[j].x >> 8) != ft) continue;
+               nab[p] = corig[j];
+               pbct[p] = code;
+               p++;
             }
          }
          nn += __shfl(inc, 63, 64);
@@ -347,7 +349,6 @@ static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *_
 
 void mgpu::run_pairlist_count(mgpu_engine *e, int, double, const int *offs, int noff,
                               long *total, long *list_start_host) {
-   ProfScope ps(e, "pairlist_count");
    std::vector<int4> h(noff);
    for (int i = 0; i < noff; i++) h[i] = make_int4(offs[3 * i], offs[3 * i + 1], offs[3 * i + 2], 0);
    e->d_off_pl.reserve(noff + 1);
@@ -356,12 +357,13 @@ void mgpu::run_pairlist_count(mgpu_engine *e, int, double, const int *offs, int 
    e->d_pl_count.reserve(nc + 1);
    e->d_pl_start.reserve(nc + 1);
    PLArgs a{e->nx, e->ny, e->nz, nc, noff, e->n_frame_types};
-   k_pairlist<false><<<(nc + 3) / 4, 256, 0, e->stream>>>(a, e->d_off_pl.p, e->d_cell_start.p,
-                                                          e->d_cell_entries.p, e->d_entry_isite.p,
-                                                          e->d_entry_num.p, e->d_entry_ftype.p,
-                                                          e->d_pl_count.p, nullptr, nullptr, nullptr,
-                                                          nullptr);
-   device_scan_int(e, e->d_pl_count.p, e->d_pl_start.p, nc);
+   {
+      ProfScope ps(e, "pairlist_count");
+      k_pairlist<false><<<(nc + 3) / 4, 256, 0, e->stream>>>(a, e->d_off_pl.p, e->d_cell_site_start.p,
+                                                             e->d_corig.p, e->d_cmeta.p, e->d_pl_count.p,
+                                                             nullptr, nullptr, nullptr, nullptr);
+      device_scan_int(e, e->d_pl_count.p, e->d_pl_start.p, nc);
+   }
    std::vector<int> hs(nc + 1);
    HIPCHK(hipMemcpyAsync(hs.data(), e->d_pl_start.p, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
@@ -390,10 +392,9 @@ void mgpu::run_pairlist_fill_device(mgpu_engine *e, int noff, long total) {
    e->d_pl_pbc.reserve(total + 1);
    e->d_pl_nfnab.reserve(2 * (size_t)nc + 2);
    PLArgs a{e->nx, e->ny, e->nz, nc, noff, e->n_frame_types};
-   k_pairlist<true><<<(nc + 3) / 4, 256, 0, e->stream>>>(
-       a, e->d_off_pl.p, e->d_cell_start.p, e->d_cell_entries.p, e->d_entry_isite.p,
-       e->d_entry_num.p, e->d_entry_ftype.p, nullptr, e->d_pl_start.p, e->d_pl_nab.p, e->d_pl_pbc.p,
-       e->d_pl_nfnab.p);
+   k_pairlist<true><<<(nc + 3) / 4, 256, 0, e->stream>>>(a, e->d_off_pl.p, e->d_cell_site_start.p,
+                                                         e->d_corig.p, e->d_cmeta.p, nullptr, e->d_pl_start.p,
+                                                         e->d_pl_nab.p, e->d_pl_pbc.p, e->d_pl_nfnab.p);
 }
 
 // ------------------------------------------------- radial distributions
