@@ -105,10 +105,9 @@ struct SkArgs {
 template <int NB>
 __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
    constexpr int NCOL = 16 * NB;
-   constexpr int BITEMS = 4 * NCOL * (KS / 8);   // (plane, column, 8-site group) half8 items
-   constexpr int BPER = (BITEMS + 255) / 256;
-   __shared__ __attribute__((aligned(16))) half_t sA[4][FROWS][KS + 8];   // Ar hi/lo, Ai hi/lo
-   __shared__ __attribute__((aligned(16))) half_t sB[4][NCOL][KS + 8];    // Zr hi/lo, Zi hi/lo
+   // A operand double-buffered in LDS: step s+1 is generated while step s
+   // is multiplied, one barrier per step
+   __shared__ __attribute__((aligned(16))) half_t sA[2][4][FROWS][KS + 8];   // Ar hi/lo, Ai hi/lo
    __shared__ int2 shk[FROWS];
    const int tile = blockIdx.x, slice = blockIdx.y;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -129,80 +128,81 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
    const size_t yoff = rvalid ? (size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp : 0;
    const float ysgn = hk.y < 0 ? -1.f : 1.f;
    float4 xr[4], yr[4];
-   half8 zr[BPER];
-   // global loads of one 32-site step into registers (issued a step ahead)
-   auto load_step = [&](int i0) {
+   auto load_xy = [&](int i0) {
       const int ib = i0 + 8 * sg;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
          xr[v] = rvalid ? *(const float4 *)(a.X + xoff + ib + 2 * v) : float4{0, 0, 0, 0};
          yr[v] = rvalid ? *(const float4 *)(a.Y + yoff + ib + 2 * v) : float4{0, 0, 0, 0};
       }
+   };
+   auto gen_a = [&](int i0, int buf) {
+      const int ib = i0 + 8 * sg;
+      half8 arh, arl, aih, ail;
 #pragma unroll
-      for (int j = 0; j < BPER; j++) {
-         const int q = tid + 256 * j;
-         half8 v = half8{0, 0, 0, 0, 0, 0, 0, 0};
-         if (q < BITEMS) {
-            const int p = q / (NCOL * 4), rem = q - p * (NCOL * 4), c = rem >> 2, gg = rem & 3;
-            const int l = c - a.lmax, al = l < 0 ? -l : l;
-            if (al <= a.lmax) {
-               const half_t *base = p == 0 ? a.Zhr : (p == 1 ? a.Zlr : (p == 2 ? a.Zhi : a.Zli));
-               v = *(const half8 *)(base + (size_t)al * a.nsp + i0 + 8 * gg);
-               if (p >= 2 && l < 0) v = -v;   // exp(-i l c*.r) = conj
-            }
+      for (int t = 0; t < 8; t++) {
+         const float4 xv = xr[t >> 1], yv = yr[t >> 1];
+         const float x0 = (t & 1) ? xv.z : xv.x, x1 = (t & 1) ? xv.w : xv.y;
+         const float y0 = (t & 1) ? yv.z : yv.x, y1 = ysgn * ((t & 1) ? yv.w : yv.y);
+         const bool ok = rvalid && ib + t >= sl.z && ib + t < sl.w;
+         const float ar = ok ? x0 * y0 - x1 * y1 : 0.f;
+         const float ai = ok ? x1 * y0 + x0 * y1 : 0.f;
+         half_t h, l;
+         split2f(ar, h, l);
+         arh[t] = h; arl[t] = l;
+         split2f(ai, h, l);
+         aih[t] = h; ail[t] = l;
+      }
+      *(half8 *)&sA[buf][0][rr][8 * sg] = arh;
+      *(half8 *)&sA[buf][1][rr][8 * sg] = arl;
+      *(half8 *)&sA[buf][2][rr][8 * sg] = aih;
+      *(half8 *)&sA[buf][3][rr][8 * sg] = ail;
+   };
+   // B operand of this wave (Zr or Zi, hi and lo) straight from the tables:
+   // lane (g, lc) takes sites 8g..8g+7 of column n*16+lc
+   const half_t *zh = wn ? a.Zhi : a.Zhr, *zl = wn ? a.Zli : a.Zlr;
+   half8 bh[NB], bl[NB];
+   auto load_b = [&](int i0) {
+#pragma unroll
+      for (int n = 0; n < NB; n++) {
+         const int l = n * 16 + lc - a.lmax, al = l < 0 ? -l : l;
+         half8 h = half8{0, 0, 0, 0, 0, 0, 0, 0}, lo = h;
+         if (al <= a.lmax) {
+            const size_t o = (size_t)al * a.nsp + i0 + 8 * g;
+            h = *(const half8 *)(zh + o);
+            lo = *(const half8 *)(zl + o);
+            if (wn && l < 0) { h = -h; lo = -lo; }   // exp(-i l c*.r) = conj
          }
-         zr[j] = v;
+         bh[n] = h;
+         bl[n] = lo;
       }
    };
    int i0 = sl.x;
-   if (i0 < sl.y) load_step(i0);
-   for (; i0 < sl.y; i0 += KS) {
-      __syncthreads();
-      {
-         const int ib = i0 + 8 * sg;
-         half8 arh, arl, aih, ail;
+   if (i0 < sl.y) {
+      load_xy(i0);
+      gen_a(i0, 0);
+      if (i0 + KS < sl.y) load_xy(i0 + KS);
+   }
+   __syncthreads();
+   for (int s = 0; i0 < sl.y; i0 += KS, s++) {
+      const int buf = s & 1;
+      load_b(i0);
+      if (i0 + KS < sl.y) {
+         gen_a(i0 + KS, buf ^ 1);
+         if (i0 + 2 * KS < sl.y) load_xy(i0 + 2 * KS);
+      }
 #pragma unroll
-         for (int t = 0; t < 8; t++) {
-            const float4 xv = xr[t >> 1], yv = yr[t >> 1];
-            const float x0 = (t & 1) ? xv.z : xv.x, x1 = (t & 1) ? xv.w : xv.y;
-            const float y0 = (t & 1) ? yv.z : yv.x, y1 = ysgn * ((t & 1) ? yv.w : yv.y);
-            const bool ok = rvalid && ib + t >= sl.z && ib + t < sl.w;
-            const float ar = ok ? x0 * y0 - x1 * y1 : 0.f;
-            const float ai = ok ? x1 * y0 + x0 * y1 : 0.f;
-            half_t h, l;
-            split2f(ar, h, l);
-            arh[t] = h; arl[t] = l;
-            split2f(ai, h, l);
-            aih[t] = h; ail[t] = l;
-         }
-         *(half8 *)&sA[0][rr][8 * sg] = arh;
-         *(half8 *)&sA[1][rr][8 * sg] = arl;
-         *(half8 *)&sA[2][rr][8 * sg] = aih;
-         *(half8 *)&sA[3][rr][8 * sg] = ail;
+      for (int m = 0; m < 4; m++) {
+         const half8 ah = *(const half8 *)&sA[buf][2 * wm][m * 16 + lc][8 * g];
+         const half8 al8 = *(const half8 *)&sA[buf][2 * wm + 1][m * 16 + lc][8 * g];
 #pragma unroll
-         for (int j = 0; j < BPER; j++) {
-            const int q = tid + 256 * j;
-            if (q < BITEMS) {
-               const int p = q / (NCOL * 4), rem = q - p * (NCOL * 4), c = rem >> 2, gg = rem & 3;
-               *(half8 *)&sB[p][c][8 * gg] = zr[j];
-            }
+         for (int n = 0; n < NB; n++) {
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8, bh[n], acc[m][n], 0, 0, 0);
          }
       }
-      if (i0 + KS < sl.y) load_step(i0 + KS);
       __syncthreads();
-#pragma unroll
-      for (int n = 0; n < NB; n++) {
-         const half8 bh = *(const half8 *)&sB[2 * wn][n * 16 + lc][8 * g];
-         const half8 bl = *(const half8 *)&sB[2 * wn + 1][n * 16 + lc][8 * g];
-#pragma unroll
-         for (int m = 0; m < 4; m++) {
-            const half8 ah = *(const half8 *)&sA[2 * wm][m * 16 + lc][8 * g];
-            const half8 al8 = *(const half8 *)&sA[2 * wm + 1][m * 16 + lc][8 * g];
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8, bh, acc[m][n], 0, 0, 0);
-         }
-      }
    }
    float *out = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wm * 2 + wn) * FROWS * NCOL;
 #pragma unroll
@@ -429,6 +429,10 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    __syncthreads();
    for (int rt = rt0; rt < ntile; rt++) {
       const int buf = (rt - rt0) & 1;
+      // this tile's row metadata, in flight during the MFMAs
+      const int r = rt * 16 + lc;
+      const int2 hk = a.rowhk[r];
+      const float4 kv = a.rowk[r];
       if (EARLY && rt + 1 < ntile) wload(rt + 1);
       floatx4 acc[2][4];
 #pragma unroll
@@ -449,10 +453,7 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
             }
          }
       }
-      const int r = rt * 16 + lc;
       if (r < a.nrows) {
-         const int2 hk = a.rowhk[r];
-         const float4 kv = a.rowk[r];
          const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
          const int ak = hk.y < 0 ? -hk.y : hk.y;
          const float ysg = hk.y < 0 ? -1.f : 1.f;
