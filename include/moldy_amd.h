@@ -106,18 +106,23 @@ int mgpu_last_info(const mgpu_engine *eng, mgpu_fc_info *fc, mgpu_ew_info *ew);
 /* ---- molecular dynamics on the device (do_step, src/accel.c:626-827) ----
  * The leapfrog splitting of src/leapfrog.c in FP64: centre-of-mass update
  * with the metric (h^T h)^-1, momentum half kicks, the symplectic rotation
- * of leapf_quat_b (or leapf_quat_a when nosymmetric_rot) and, when
- * const_temp, the Nose-Poincare thermostat.  The cell is fixed (no barostat).
- * Forces come from mgpu_eval_forces' device path at the engine's precision. */
+ * of leapf_quat_b (or leapf_quat_a when nosymmetric_rot), when const_temp
+ * the Nose-Poincare thermostat and, when const_pressure, the cell dynamics
+ * of gleap_cell / update_hmom (src/leapfrog.c:475-531; 1 Parrinello-Rahman,
+ * 2 uniform dilation).  Forces come from mgpu_eval_forces' device path at
+ * the engine's precision. */
 typedef struct {
    double step;                  /* control.step, program time units          */
    double ttmass;                /* control.ttmass                            */
-   double pmass;                 /* control.pmass (enters H_0 via ke_cell)    */
+   double pmass;                 /* control.pmass, the cell mass W            */
    double gkt;                   /* sys->d_of_f * kB * control.temp           */
    int const_temp;               /* 0, or nonzero = Nose-Poincare thermostat  */
    int nosymmetric_rot;          /* control.nosymmetric_rot                   */
    const double *spec_mass;      /* [nspecies] molecular masses               */
    const double *spec_inertia;   /* [nspecies][3] principal moments           */
+   int const_pressure;           /* control.const_pressure: 0, 1 or 2         */
+   double pressure;              /* control.pressure, program units           */
+   int strain_mask;              /* control.strain_mask: bit 3i+j fixes h_ij  */
 } mgpu_dynamics;
 
 int mgpu_set_dynamics(mgpu_engine *eng, const mgpu_dynamics *dyn);
@@ -128,13 +133,21 @@ int mgpu_upload_momenta(mgpu_engine *eng, const double *mom, const double *amom,
                         const double thermo[3]);
 /* Runs nsteps do_steps on the device.  init_H_0: evaluate H_0 on the first
  * step (accel.c:718-724).  use_graph: replay steps 2..nsteps from a captured
- * hipGraph.  trace (optional, [nsteps][8]) receives per step: translational
- * KE, rotational KE, pe[0], pe[1] at the H_0 point, s, p_s, H_0, 0.  pe and
- * stress (optional) are those of the last step's force evaluation. */
+ * hipGraph (ignored under const_pressure: the cell, and with it the link
+ * cells and the k-vector set, changes every step, so each step is launched
+ * eagerly after the host reads the new h).  trace (optional, [nsteps][8])
+ * receives per step: translational KE, rotational KE, pe[0], pe[1] at the
+ * H_0 point, s, p_s, H_0, det(h).  pe and stress (optional) are those of the
+ * last step's force evaluation. */
 int mgpu_md_steps(mgpu_engine *eng, int nsteps, int init_H_0, int use_graph, double *trace,
                   double pe[2], double stress[9]);
 int mgpu_download_dynamics(mgpu_engine *eng, double *c_of_m_frac, double *quat, double *mom,
                            double *amom, double thermo[3]);
+/* The cell momentum sys->hmom[3][3] (zero after mgpu_set_dynamics), and the
+ * cell h with its momentum after a run (h as advanced by the barostat; it
+ * replaces the h of mgpu_upload_state for later calls). */
+int mgpu_upload_cell_momentum(mgpu_engine *eng, const double hmom[9]);
+int mgpu_download_cell(mgpu_engine *eng, double h[9], double hmom[9]);
 
 /* ---- link cells and pair lists (bit-exact with src/force.c:426-569) ----
  * mgpu_cells: per entry (a site, or a molecule when molpbc) its cell, and the

@@ -113,6 +113,7 @@ void capture_batch(mgpu_batch *b) {
 // Enqueue nsteps of every replica under schedule s (no host sync).
 void enqueue_steps(mgpu_batch *b, int nsteps, bool init_h0, int s) {
    for (auto *e : b->eng) {
+      md_load_cell(e);
       md_reset_record(e, nsteps);
       zero_async(e, e->d_iflags.p, sizeof(int) * 48);
    }
@@ -263,6 +264,10 @@ int mgpu_batch_add(mgpu_batch *b, mgpu_engine *e) {
       }
       if (e->device != b->device) {
          b->err = "engine on another device";
+         return (int)MGPU_E_ARG;
+      }
+      if (e->dyn_const_pressure) {
+         b->err = "a barostat engine re-plans its cell every step: run it with mgpu_md_steps";
          return (int)MGPU_E_ARG;
       }
       b->eng.push_back(e);

@@ -1,14 +1,17 @@
 // Molecular-dynamics step on the device: do_step of src/accel.c:626-827 with
 // the leapfrog splitting of src/leapfrog.c (centre-of-mass update with the
 // metric G^-1, momentum kicks, the symplectic quaternion rotations of
-// leapf_quat_a/leapf_quat_b) and the Nose-Poincare thermostat, in FP64.
-// The cell matrix is constant (no barostat), so every launch of a step has
-// fixed arguments and the whole step can be captured in a hipGraph.
+// leapf_quat_a/leapf_quat_b), the Nose-Poincare thermostat and the
+// Parrinello-Rahman / uniform-dilation barostat, in FP64.  At constant cell
+// every launch of a step has fixed arguments and the whole step can be
+// captured in a hipGraph; under the barostat the host reads the new h after
+// the first cell half step and rebuilds the cell-dependent plans.
 //
 // Device layout: one thread per molecule; momenta mom[nmols][3] (scaled,
 // mom = h^T p), principal angular momenta amom[nmols_r][4]; the integrator
-// scalars (s, p_s, H_0, record counter) and per-species kinetic sums live
-// in e->d_scal so that no value goes back to the host inside a step.
+// scalars (s, p_s, H_0, record counter, h, hmom) and per-species kinetic
+// sums live in e->d_scal so that no value goes back to the host inside a
+// constant-cell step.
 #include "mgpu_internal.h"
 
 #include <algorithm>
@@ -22,11 +25,12 @@ constexpr double INERTIA_MIN = 1.0e-14;   // src/leapfrog.c:75
 constexpr double QTOL = 1.0e-4;           // src/leapfrog.c:72
 constexpr double DMIN = 2.2250738585072014e-308;
 
-// d_scal slots
-enum { S_TS = 0, S_TSMOM = 1, S_H0 = 2, S_ROW = 3, S_KE = 8 };
-constexpr int MAX_SPEC = 14;              // S_KE + 4 * MAX_SPEC <= 64
-constexpr int SCAL_SIZE = 64;
-constexpr int TRACE_W = 8;                // ke_t, ke_r, pe0, pe1, ts, tsmom, H_0, 0
+// d_scal slots: thermostat, record row, per-species KE sums (4 a species),
+// cell h and hmom (row-major), per-species momentum dyads (6 a species)
+enum { S_TS = 0, S_TSMOM = 1, S_H0 = 2, S_ROW = 3, S_KE = 8, S_H = 64, S_HMOM = 73, S_DY = 82 };
+constexpr int MAX_SPEC = 14;              // S_KE + 4 * MAX_SPEC <= S_H
+constexpr int SCAL_SIZE = 192;            // S_DY + 6 * MAX_SPEC <= SCAL_SIZE
+constexpr int TRACE_W = 8;                // ke_t, ke_r, pe0, pe1, ts, tsmom, H_0, det(h)
 
 struct SpecDyn {
    double mass, I[3], rI[3], fw;
@@ -223,13 +227,43 @@ __global__ __launch_bounds__(256) void k_ke_part(KeArgs a, const int *__restrict
           ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
 }
 
-// Column sums of the partials into scal[S_KE ..], one thread per column.
-__global__ void k_ke_sum(int nblocks, int nc, const double *__restrict__ part, double *__restrict__ scal) {
+// Per-species sums of the real-momentum dyad p p^T (energy_dyad,
+// src/algorith.c:261-285): columns 6s + {xx, xy, xz, yy, yz, zz}.
+__global__ __launch_bounds__(256) void k_dyad_part(KeArgs a, const int *__restrict__ mol_spec,
+                                                   const double *__restrict__ mom, double *__restrict__ part) {
+   __shared__ double ws[4][6 * MAX_SPEC];
+   int m = blockIdx.x * 256 + threadIdx.x;
+   double v[6] = {0, 0, 0, 0, 0, 0};
+   int sp = -1;
+   if (m < a.nmols) {
+      sp = mol_spec[m];
+      const double *q = mom + 3 * m;
+      double p0 = a.hiT[0] * q[0] + a.hiT[1] * q[1] + a.hiT[2] * q[2];
+      double p1 = a.hiT[3] * q[0] + a.hiT[4] * q[1] + a.hiT[5] * q[2];
+      double p2 = a.hiT[6] * q[0] + a.hiT[7] * q[1] + a.hiT[8] * q[2];
+      v[0] = p0 * p0; v[1] = p0 * p1; v[2] = p0 * p2;
+      v[3] = p1 * p1; v[4] = p1 * p2; v[5] = p2 * p2;
+   }
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   for (int s = 0; s < a.nspec; s++)
+      for (int k = 0; k < 6; k++) {
+         double t = wave_sum(sp == s ? v[k] : 0.0);
+         if (lane == 0) ws[w][6 * s + k] = t;
+      }
+   __syncthreads();
+   const int nc = 6 * a.nspec;
+   if (threadIdx.x < nc)
+      part[(size_t)blockIdx.x * nc + threadIdx.x] =
+          ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+// Column sums of the partials into out[0 .. nc), one thread per column.
+__global__ void k_col_sum(int nblocks, int nc, const double *__restrict__ part, double *__restrict__ out) {
    int c = threadIdx.x;
    if (c >= nc) return;
    double t = 0.0;
    for (int b = 0; b < nblocks; b++) t += part[(size_t)b * nc + c];
-   scal[S_KE + c] = t;
+   out[c] = t;
 }
 
 enum {
@@ -240,7 +274,10 @@ enum {
    OP_TSMOM_AXES,       // tsmom += w * stepdts2 * rI_k * sum amom_k^2 (leapfrog.c:364-381)
    OP_RECORD,           // KE at the H_0 point, H_0, trace row (accel.c:714-724)
    OP_TSMOM_PE,         // tsmom -= step (pe - H_0) (accel.c:759-760)
-   OP_RESET_ROW
+   OP_RESET_ROW,
+   OP_GLEAP_CELL,       // gleap_cell at a half step (accel.c:688-690, 777-779)
+   OP_HMOM_KIN,         // update_hmom with stress_kin at a half step (accel.c:700-704, 767-771)
+   OP_HMOM_VIR          // update_hmom with the virial at a full step (accel.c:760-761)
 };
 
 struct ScalArgs {
@@ -248,7 +285,75 @@ struct ScalArgs {
    double step, ttmass, pmass, gkt;
    double w[3];                 // per-axis weights (OP_TSMOM_AXES), order of addition
    int axes[3], naxes;
+   int const_temp, const_pressure, uniform, strain_mask;
+   double pressure;
 };
+
+// 3x3 helpers in the index order of src/algorith.c (det, mk_sigma,
+// trace_sqr) and src/leapfrog.c (ke_cell).
+__device__ double det3(const double *a) {
+   double d = 0.0;
+   for (int i = 0, j = 1, k = 2; i < 3; i++, j = (j + 1) % 3, k = (k + 1) % 3)
+      d += a[i] * (a[3 + j] * a[6 + k] - a[3 + k] * a[6 + j]);
+   return d;
+}
+
+__device__ void mk_sigma(const double *h, double *sg) {
+   for (int i = 0, j = 1, k = 2; i < 3; i++, j = (j + 1) % 3, k = (k + 1) % 3)
+      for (int l = 0, m = 1, n = 2; l < 3; l++, m = (m + 1) % 3, n = (n + 1) % 3)
+         sg[3 * i + l] = h[3 * j + m] * h[3 * k + n] - h[3 * k + m] * h[3 * j + n];
+}
+
+__device__ double sum_sq9(const double *m) {
+   double t = 0.0;
+   for (int i = 0; i < 9; i++) t += m[i] * m[i];
+   return t;
+}
+
+// leapf_hmom, src/leapfrog.c:415-430: pressure term, strain mask in
+// row-major bit order
+__device__ void leapf_hmom(double step, double *hmom, const double *sigma, double s, double pressure, int mask) {
+   const double r = 0.5 * step * s * pressure;
+   for (int i = 0; i < 9; i++) {
+      if (mask & 1) hmom[i] = 0.0;
+      else hmom[i] -= r * sigma[i];
+      mask >>= 1;
+   }
+}
+
+__device__ void hmom_pressure(const ScalArgs &a, double step, double s, const double *h, double *hmom) {
+   if (a.uniform) {
+      const double vol = det3(h);
+      leapf_hmom(step, hmom, h, 3.0 * vol * s / sum_sq9(h), a.pressure, a.strain_mask);
+   } else {
+      double sg[9];
+      mk_sigma(h, sg);
+      leapf_hmom(step, hmom, sg, s, a.pressure, a.strain_mask);
+   }
+}
+
+// update_hmom, src/leapfrog.c:516-531; sp is a stress times the volume
+__device__ void update_hmom(const ScalArgs &a, double step, double s, const double *h, const double *sp,
+                            double *hmom) {
+   double t[9];
+   if (a.uniform) {
+      const double c = step * s * (sp[0] + sp[4] + sp[8]) / sum_sq9(h);
+      for (int i = 0; i < 9; i++) t[i] = c * h[i];
+   } else {
+      double sg[9];
+      const double vol = det3(h);
+      mk_sigma(h, sg);
+      for (int i = 0; i < 3; i++)
+         for (int j = 0; j < 3; j++) {
+            double x = 0.0;
+            for (int k = 0; k < 3; k++) x += sp[3 * i + k] * sg[3 * k + j];
+            t[3 * i + j] = x;
+         }
+      const double c = step * s / vol;
+      for (int i = 0; i < 9; i++) t[i] *= c;
+   }
+   for (int i = 0; i < 9; i++) hmom[i] += t[i];
+}
 
 __device__ double ke_trans(const ScalArgs &a, const SpecDyn *sd, const double *scal) {
    double ts = scal[S_TS], ke = 0.0;
@@ -296,20 +401,47 @@ __global__ void k_scalar(ScalArgs a, const SpecDyn *__restrict__ sd, const int *
          for (int s = 0; s < a.nspec; s++)
             if (spec_rdof[s] > 0) kr += ke_rot(a, sd, scal, s);
          const double pe0 = res[0], pe1 = res[1];
-         if (a.init_h0)
+         const double vol = det3(scal + S_H);
+         if (a.init_h0) {
             scal[S_H0] = (kt + kr) + (pe0 + pe1) + tsmom * tsmom / (2.0 * a.ttmass) + a.gkt * log(ts) +
-                         0.5 * 0.0 / a.pmass;
+                         0.5 / a.pmass * sum_sq9(scal + S_HMOM);
+            if (a.const_pressure) scal[S_H0] += a.pressure * vol;
+         }
          int row = (int)scal[S_ROW];
          if (row < a.trace_cap) {
             double *t = trace + (size_t)row * TRACE_W;
             t[0] = kt; t[1] = kr; t[2] = pe0; t[3] = pe1; t[4] = ts; t[5] = tsmom; t[6] = scal[S_H0];
-            t[7] = 0.0;
+            t[7] = vol;
          }
          scal[S_ROW] = row + 1;
          break;
       }
       case OP_TSMOM_PE: tsmom -= a.step * (res[0] + res[1] - scal[S_H0]); break;
       case OP_RESET_ROW: scal[S_ROW] = 0.0; break;
+      case OP_GLEAP_CELL: {
+         // gleap_cell, src/leapfrog.c:475-508, at step = a.step / 2
+         const double st = 0.5 * a.step;
+         double *h = scal + S_H, *hm = scal + S_HMOM;
+         hmom_pressure(a, st, ts, h, hm);
+         if (a.const_temp) tsmom -= 0.5 * st * (0.5 / a.pmass * sum_sq9(hm) + a.pressure * det3(h));
+         const double r = st * ts / a.pmass;   // leapf_h
+         for (int i = 0; i < 9; i++) h[i] += r * hm[i];
+         if (a.const_temp) tsmom -= 0.5 * st * (0.5 / a.pmass * sum_sq9(hm) + a.pressure * det3(h));
+         hmom_pressure(a, st, ts, h, hm);
+         break;
+      }
+      case OP_HMOM_KIN: {
+         // stress_kin (accel.c:349-355): sum over species of dyad / (m s^2)
+         double kd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+         const int ij[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+         for (int s = 0; s < a.nspec; s++) {
+            const double d = sd[s].mass * ts * ts;
+            for (int q = 0; q < 9; q++) kd[q] += scal[S_DY + 6 * s + ij[q]] / d;
+         }
+         update_hmom(a, 0.5 * a.step, ts, scal + S_H, kd, scal + S_HMOM);
+         break;
+      }
+      case OP_HMOM_VIR: update_hmom(a, a.step, ts, scal + S_H, res + 5, scal + S_HMOM); break;
    }
 }
 
@@ -331,6 +463,11 @@ ScalArgs scal_args(mgpu_engine *e, int op) {
    a.ttmass = e->dyn_ttmass;
    a.pmass = e->dyn_pmass;
    a.gkt = e->dyn_gkt;
+   a.const_temp = e->dyn_const_temp;
+   a.const_pressure = e->dyn_const_pressure;
+   a.uniform = e->dyn_const_pressure > 0 && e->dyn_const_pressure % 2 == 0;   // accel.c:657
+   a.strain_mask = e->dyn_strain_mask;
+   a.pressure = e->dyn_pressure;
    return a;
 }
 
@@ -339,18 +476,26 @@ void scalar_op(mgpu_engine *e, const ScalArgs &a) {
                                      e->d_out.p + OUT_RES, e->d_scal.p, e->d_trace.p);
 }
 
-void ke_sums(mgpu_engine *e) {
+KeArgs ke_args(mgpu_engine *e) {
    KeArgs k;
    k.nmols = e->nmols;
    k.nspec = e->nspecies;
-   double hiT[9];
    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) hiT[3 * i + j] = e->hinv[3 * j + i];
-   for (int i = 0; i < 9; i++) k.hiT[i] = hiT[i];
+      for (int j = 0; j < 3; j++) k.hiT[3 * i + j] = e->hinv[3 * j + i];
+   return k;
+}
+
+void ke_sums(mgpu_engine *e) {
    const int nb = (e->nmols + 255) / 256, nc = 4 * e->nspecies;
-   k_ke_part<<<nb, 256, 0, e->stream>>>(k, e->d_mol_spec.p, e->d_mol_quat.p, e->d_mom.p, e->d_amom.p,
+   k_ke_part<<<nb, 256, 0, e->stream>>>(ke_args(e), e->d_mol_spec.p, e->d_mol_quat.p, e->d_mom.p, e->d_amom.p,
                                         e->d_red.p);
-   k_ke_sum<<<1, 64, 0, e->stream>>>(nb, nc, e->d_red.p, e->d_scal.p);
+   k_col_sum<<<1, 128, 0, e->stream>>>(nb, nc, e->d_red.p, e->d_scal.p + S_KE);
+}
+
+void dyad_sums(mgpu_engine *e) {
+   const int nb = (e->nmols + 255) / 256, nc = 6 * e->nspecies;
+   k_dyad_part<<<nb, 256, 0, e->stream>>>(ke_args(e), e->d_mol_spec.p, e->d_mom.p, e->d_red.p);
+   k_col_sum<<<1, 128, 0, e->stream>>>(nb, nc, e->d_red.p, e->d_scal.p + S_DY);
 }
 
 // One leapf_all_coords half step, with the thermostat's rotational terms
@@ -430,6 +575,9 @@ void mgpu::md_setup(mgpu_engine *e, const mgpu_dynamics *d) {
    if (e->nspecies > MAX_SPEC) throw StatusError{MGPU_E_CAPACITY, "more than 14 species in the MD step"};
    if (!(d->step > 0)) throw StatusError{MGPU_E_ARG, "step must be positive"};
    if (d->const_temp && !(d->ttmass > 0)) throw StatusError{MGPU_E_ARG, "ttmass must be positive"};
+   if (d->const_pressure < 0 || d->const_pressure > 2)
+      throw StatusError{MGPU_E_ARG, "const_pressure must be 0, 1 or 2"};
+   if (!(d->pmass > 0)) throw StatusError{MGPU_E_ARG, "pmass must be positive"};
    std::vector<SpecDyn> sd(e->nspecies);
    e->spec_mass.assign(d->spec_mass, d->spec_mass + e->nspecies);
    e->spec_inertia.assign(d->spec_inertia, d->spec_inertia + 3 * e->nspecies);
@@ -464,6 +612,10 @@ void mgpu::md_setup(mgpu_engine *e, const mgpu_dynamics *d) {
    e->dyn_gkt = d->gkt;
    e->dyn_const_temp = d->const_temp != 0;
    e->dyn_nosym = d->nosymmetric_rot != 0;
+   e->dyn_const_pressure = d->const_pressure;
+   e->dyn_pressure = d->pressure;
+   e->dyn_strain_mask = d->strain_mask;
+   for (int i = 0; i < 9; i++) e->hmom[i] = 0.0;
    e->dyn_saxis = saxis;
    e->d_spec_dyn.reserve(sizeof(SpecDyn) / sizeof(double) * e->nspecies);
    HIPCHK(hipMemcpy(e->d_spec_dyn.p, sd.data(), sizeof(SpecDyn) * sd.size(), hipMemcpyHostToDevice));
@@ -471,7 +623,7 @@ void mgpu::md_setup(mgpu_engine *e, const mgpu_dynamics *d) {
    e->d_amom.reserve(4 * (size_t)std::max(e->nmols_r, 1));
    e->d_scal.reserve(SCAL_SIZE);
    const int nb = (e->nmols + 255) / 256;
-   e->d_red.reserve((size_t)nb * 4 * MAX_SPEC + 64);
+   e->d_red.reserve((size_t)nb * 6 * MAX_SPEC + 64);
    e->have_dyn = true;
 }
 
@@ -480,8 +632,33 @@ void mgpu::md_set_scalars(mgpu_engine *e, const double thermo[3]) {
    v[S_TS] = thermo[0];
    v[S_TSMOM] = thermo[1];
    v[S_H0] = thermo[2];
+   for (int i = 0; i < 9; i++) {
+      v[S_H + i] = e->h[i];
+      v[S_HMOM + i] = e->hmom[i];
+   }
    HIPCHK(hipMemcpyAsync(e->d_scal.p, v, sizeof v, hipMemcpyHostToDevice, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
+}
+
+// The host's h and hmom into the device slots (start of a run).
+void mgpu::md_load_cell(mgpu_engine *e) {
+   double v[18];
+   for (int i = 0; i < 9; i++) {
+      v[i] = e->h[i];
+      v[9 + i] = e->hmom[i];
+   }
+   HIPCHK(hipMemcpyAsync(e->d_scal.p + S_H, v, sizeof v, hipMemcpyHostToDevice, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+}
+
+void mgpu::md_fetch_cell(mgpu_engine *e, double h[9], double hmom[9]) {
+   double v[18];
+   HIPCHK(hipMemcpyAsync(v, e->d_scal.p + S_H, sizeof v, hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   for (int i = 0; i < 9; i++) {
+      if (h) h[i] = v[i];
+      if (hmom) hmom[i] = v[9 + i];
+   }
 }
 
 void mgpu::md_get_scalars(mgpu_engine *e, double thermo[3]) {
@@ -522,6 +699,43 @@ void mgpu::md_step_launch(mgpu_engine *e, bool init_h0) {
       scalar_op(e, scal_args(e, OP_TSMOM_TRANS));
    }
    coords_half(e);
+   if (therm) scalar_op(e, scal_args(e, OP_THERM_OUT));
+}
+
+// do_step with the barostat (const_pressure branches of src/accel.c:686-779).
+// The cell half step runs on the device; the host then reads h, rebuilds the
+// link cells and the k-vector set for it and enqueues the rest of the step.
+void mgpu::md_step_pressure(mgpu_engine *e, bool init_h0) {
+   const bool therm = e->dyn_const_temp;
+   if (therm) scalar_op(e, scal_args(e, OP_THERM_IN));
+   scalar_op(e, scal_args(e, OP_GLEAP_CELL));
+   double h[9];
+   md_fetch_cell(e, h, nullptr);
+   set_geometry(e, h);
+   if (therm) {
+      ke_sums(e);
+      scalar_op(e, scal_args(e, OP_TSMOM_TRANS));
+   }
+   dyad_sums(e);
+   scalar_op(e, scal_args(e, OP_HMOM_KIN));
+   coords_half(e);
+   eval_forces_device(e, false);
+   momenta_half(e, false);
+   ke_sums(e);
+   ScalArgs r = scal_args(e, OP_RECORD);
+   r.init_h0 = init_h0;
+   scalar_op(e, r);
+   momenta_half(e, true);
+   if (therm) scalar_op(e, scal_args(e, OP_TSMOM_PE));
+   scalar_op(e, scal_args(e, OP_HMOM_VIR));
+   if (therm) {
+      ke_sums(e);
+      scalar_op(e, scal_args(e, OP_TSMOM_TRANS));
+   }
+   dyad_sums(e);
+   scalar_op(e, scal_args(e, OP_HMOM_KIN));
+   coords_half(e);
+   scalar_op(e, scal_args(e, OP_GLEAP_CELL));
    if (therm) scalar_op(e, scal_args(e, OP_THERM_OUT));
 }
 

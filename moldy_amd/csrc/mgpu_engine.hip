@@ -200,9 +200,9 @@ static int neighbour_offsets(const double h[9], double cutoff, int nx, int ny, i
 }
 #pragma clang fp contract(on)
 
-namespace {
-
-void set_geometry(mgpu_engine *e, const double h[9]) {
+// Cell-dependent host state: link-cell grid and neighbour offsets, the
+// k-vector set; bumps geom_version when either changes.
+void mgpu::set_geometry(mgpu_engine *e, const double h[9]) {
 #pragma clang fp contract(off)
    memcpy(e->h, h, 9 * sizeof(double));
    host_invert(e->h, e->hinv);
@@ -312,6 +312,8 @@ void set_geometry(mgpu_engine *e, const double h[9]) {
       }
    }
 }
+
+namespace {
 
 void compute_statics(mgpu_engine *e, const double *chg) {
    // eintra, src/force.c:654-668 + 1158-1168 (charges of each species' first molecule)
@@ -554,8 +556,6 @@ void mgpu::eval_forces_device(mgpu_engine *e, bool clear_flags) {
    run_glue(e);
 }
 
-namespace {
-}  // namespace
 
 // Pre-existing site_force contributes sum r_i F_i to the reference's virial
 // (force.c:974-982 sums over the accumulated array).
@@ -909,15 +909,24 @@ int mgpu_md_steps(mgpu_engine *e, int nsteps, int init_h0, int use_graph, double
       if (nsteps < 0) return fail(e, MGPU_E_ARG, "nsteps < 0");
       if (nsteps == 0) return 0;
       HIPCHK(hipSetDevice(e->device));
+      md_load_cell(e);
       md_reset_record(e, nsteps);
       zero_async(e, e->d_iflags.p, sizeof(int) * 48);
-      md_step_launch(e, init_h0 != 0);
-      int done = 1;
-      if (use_graph && nsteps > 1) {
-         ensure_step_graph(e);
-         for (; done < nsteps; done++) HIPCHK(hipGraphLaunch(e->graph_exec, e->stream));
+      if (e->dyn_const_pressure) {
+         // the cell moves: eager steps, the host re-plans after each cell update
+         for (int k = 0; k < nsteps; k++) md_step_pressure(e, k == 0 && init_h0);
+         double h[9];
+         md_fetch_cell(e, h, e->hmom);
+         set_geometry(e, h);
+      } else {
+         md_step_launch(e, init_h0 != 0);
+         int done = 1;
+         if (use_graph && nsteps > 1) {
+            ensure_step_graph(e);
+            for (; done < nsteps; done++) HIPCHK(hipGraphLaunch(e->graph_exec, e->stream));
+         }
+         for (; done < nsteps; done++) md_step_launch(e, false);
       }
-      for (; done < nsteps; done++) md_step_launch(e, false);
       if (trace)
          HIPCHK(hipMemcpyAsync(trace, e->d_trace.p, sizeof(double) * md_trace_width() * nsteps,
                                hipMemcpyDeviceToHost, e->stream));
@@ -947,6 +956,26 @@ int mgpu_download_dynamics(mgpu_engine *e, double *cm, double *quat, double *mom
       double th[3];
       md_get_scalars(e, th);
       if (thermo) for (int k = 0; k < 3; k++) thermo[k] = th[k];
+      return 0;
+   });
+}
+
+int mgpu_upload_cell_momentum(mgpu_engine *e, const double hmom[9]) {
+   return guarded(e, [&] {
+      if (!e->have_dyn) return fail(e, MGPU_E_STATE, "mgpu_set_dynamics not called");
+      if (!hmom) return fail(e, MGPU_E_ARG, "bad buffers");
+      for (int i = 0; i < 9; i++) e->hmom[i] = hmom[i];
+      return 0;
+   });
+}
+
+int mgpu_download_cell(mgpu_engine *e, double h[9], double hmom[9]) {
+   return guarded(e, [&] {
+      if (!e->have_system) return fail(e, MGPU_E_STATE, "mgpu_set_system not called");
+      for (int i = 0; i < 9; i++) {
+         if (h) h[i] = e->h[i];
+         if (hmom) hmom[i] = e->hmom[i];
+      }
       return 0;
    });
 }

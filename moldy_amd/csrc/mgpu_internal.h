@@ -186,6 +186,9 @@ struct mgpu_engine {
    unsigned long graph_epoch = 0;
    double dyn_step = 0, dyn_ttmass = 0, dyn_pmass = 0, dyn_gkt = 0;
    int dyn_const_temp = 0, dyn_nosym = 0, dyn_saxis = 0;
+   int dyn_const_pressure = 0, dyn_strain_mask = 0;
+   double dyn_pressure = 0;
+   double hmom[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // cell momentum between runs
    std::vector<double> spec_mass, spec_inertia;
 
    // ---- diagnostics ----
@@ -247,6 +250,10 @@ void md_set_scalars(mgpu_engine *e, const double thermo[3]);
 void md_get_scalars(mgpu_engine *e, double thermo[3]);
 void md_reset_record(mgpu_engine *e, int cap);
 void md_step_launch(mgpu_engine *e, bool init_h0);
+void md_load_cell(mgpu_engine *e);
+void md_step_pressure(mgpu_engine *e, bool init_h0);
+void md_fetch_cell(mgpu_engine *e, double h[9], double hmom[9]);
+void set_geometry(mgpu_engine *e, const double h[9]);
 void ensure_step_graph(mgpu_engine *e);
 void autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart);
 void handle_flags(mgpu_engine *e, const int *fl);   // throws on error flags

@@ -52,7 +52,8 @@ class FcInfo(C.Structure):
 class MgpuDynamics(C.Structure):
     _fields_ = [("step", C.c_double), ("ttmass", C.c_double), ("pmass", C.c_double),
                 ("gkt", C.c_double), ("const_temp", C.c_int), ("nosymmetric_rot", C.c_int),
-                ("spec_mass", _dp), ("spec_inertia", _dp)]
+                ("spec_mass", _dp), ("spec_inertia", _dp), ("const_pressure", C.c_int),
+                ("pressure", C.c_double), ("strain_mask", C.c_int)]
 
 
 class EwInfo(C.Structure):
@@ -280,7 +281,8 @@ class ForceEngine:
         self.dyn = MgpuDynamics(dyn.step, dyn.ttmass, dyn.pmass,
                                 s.degrees_of_freedom() * units.KB * dyn.temp,
                                 int(dyn.const_temp), int(dyn.nosymmetric_rot),
-                                _p(self._dyn_arrays[0]), _p(self._dyn_arrays[1]))
+                                _p(self._dyn_arrays[0]), _p(self._dyn_arrays[1]),
+                                int(dyn.const_pressure), float(dyn.pressure), int(dyn.strain_mask))
         self._chk(lib().mgpu_set_dynamics(self.h_, C.byref(self.dyn)))
 
     def upload_momenta(self, mom, amom, ts=1.0, tsmom=0.0, H_0=0.0):
@@ -308,6 +310,16 @@ class ForceEngine:
         self._chk(lib().mgpu_download_dynamics(self.h_, _p(cm), _p(q), _p(mom), _p(amom), _p(th)))
         return dict(c_of_m=cm, quat=q[:s.nmols_r], mom=mom, amom=amom[:s.nmols_r], ts=th[0],
                     tsmom=th[1], H_0=th[2])
+
+    def upload_cell_momentum(self, hmom):
+        hm = np.ascontiguousarray(hmom, np.float64).reshape(9)
+        self._chk(lib().mgpu_upload_cell_momentum(self.h_, _p(hm)))
+
+    def download_cell(self):
+        """(h, hmom) as 3x3 arrays: the cell after the last run."""
+        h, hm = np.zeros(9), np.zeros(9)
+        self._chk(lib().mgpu_download_cell(self.h_, _p(h), _p(hm)))
+        return h.reshape(3, 3), hm.reshape(3, 3)
 
     # ------------------------------------------------ cells and pair lists
     def cells(self, h, xyz, c_of_m=None):

@@ -56,6 +56,9 @@ class Dynamics:
     ttmass: float = 100.0
     pmass: float = 100.0
     nosymmetric_rot: bool = False  # True: leapf_quat_a splitting
+    const_pressure: int = 0        # 1: Parrinello-Rahman, 2: uniform dilation
+    pressure: float = 0.0          # program units
+    strain_mask: int = 200         # bit 3i+j set: h_ij held fixed (200: lower triangle)
 
 
 @dataclasses.dataclass

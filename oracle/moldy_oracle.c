@@ -1624,11 +1624,137 @@ static void orc_nose_therm(double step, double mass, double *s, double *smom)
    *s *= scale * scale;
 }
 
-int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_state *st,
-                int init_H_0, double pe[2], double stress[9], double rec[7])
+/* ---- barostat pieces, src/leapfrog.c:397-531 and src/algorith.c:261-306 ---- */
+static double orc_trace_sqr(const double m[9])
+{
+   int i;
+   double sum = 0.0;
+   for (i = 0; i < 9; i++)
+      sum += m[i] * m[i];
+   return sum;
+}
+
+static double orc_ke_cell(const double hmom[9], double w)
+{
+   double s = 0.0;
+   int i;
+   for (i = 0; i < 9; i++)
+      s += hmom[i] * hmom[i];
+   return 0.5 / w * s;
+}
+
+/* mk_sigma: sigma = vol * h^T^-1 (cofactors), src/algorith.c:287-294 */
+static void orc_mk_sigma(const double h[9], double sg[9])
+{
+   int i, j, k, l, m, n;
+   for (i = 0, j = 1, k = 2; i < 3; i++, j = (j + 1) % 3, k = (k + 1) % 3)
+      for (l = 0, m = 1, n = 2; l < 3; l++, m = (m + 1) % 3, n = (n + 1) % 3)
+         sg[3 * i + l] = h[3 * j + m] * h[3 * k + n] - h[3 * k + m] * h[3 * j + n];
+}
+
+/* leapf_hmom, src/leapfrog.c:415-430 */
+static void orc_leapf_hmom(double step, double hmom[9], const double sigma[9], double s,
+                           double pressure, int mask)
+{
+   int i;
+   double r = 0.5 * step * s * pressure;
+   for (i = 0; i < 9; i++) {
+      if (mask & 1)
+         hmom[i] = 0;
+      else
+         hmom[i] -= r * sigma[i];
+      mask >>= 1;
+   }
+}
+
+/* gleap_cell, src/leapfrog.c:479-508 */
+static void orc_gleap_cell(const orc_dyn *d, double step, double s, double h[9], double hmom[9],
+                           double *smom, int uniform)
+{
+   double vol = orc_det(h), sigma[9];
+   int i;
+   if (uniform)
+      orc_leapf_hmom(step, hmom, h, 3.0 * vol * s / orc_trace_sqr(h), d->pressure, d->strain_mask);
+   else {
+      orc_mk_sigma(h, sigma);
+      orc_leapf_hmom(step, hmom, sigma, s, d->pressure, d->strain_mask);
+   }
+   if (d->const_temp)
+      *smom -= 0.5 * step * (orc_ke_cell(hmom, d->pmass) + d->pressure * vol);
+   for (i = 0; i < 9; i++)   /* leapf_h, src/leapfrog.c:434-442 */
+      h[i] += step * s / d->pmass * hmom[i];
+   vol = orc_det(h);
+   if (d->const_temp)
+      *smom -= 0.5 * step * (orc_ke_cell(hmom, d->pmass) + d->pressure * vol);
+   if (uniform)
+      orc_leapf_hmom(step, hmom, h, 3.0 * vol * s / orc_trace_sqr(h), d->pressure, d->strain_mask);
+   else {
+      orc_mk_sigma(h, sigma);
+      orc_leapf_hmom(step, hmom, sigma, s, d->pressure, d->strain_mask);
+   }
+}
+
+/* update_hmom, src/leapfrog.c:516-531; stress_part is a virial (times V) */
+static void orc_update_hmom(double step, double s, const double h[9], const double sp[9],
+                            double hmom[9], int uniform)
+{
+   double tmp[9], sigma[9], vol, c;
+   int i, j, k;
+   if (uniform) {
+      c = step * s * (sp[0] + sp[4] + sp[8]) / orc_trace_sqr(h);
+      for (i = 0; i < 9; i++)
+         tmp[i] = c * h[i];
+   } else {
+      vol = orc_det(h);
+      orc_mk_sigma(h, sigma);
+      for (i = 0; i < 3; i++)
+         for (j = 0; j < 3; j++) {
+            tmp[3 * i + j] = 0.0;
+            for (k = 0; k < 3; k++)
+               tmp[3 * i + j] += sp[3 * i + k] * sigma[3 * k + j];
+         }
+      c = step * s / vol;
+      for (i = 0; i < 9; i++)
+         tmp[i] *= c;
+   }
+   for (i = 0; i < 9; i++)
+      hmom[i] += tmp[i];
+}
+
+/* stress_kin / energy_dyad, src/accel.c:349-355, src/algorith.c:261-285 */
+static void orc_stress_kin(const orc_sys *s, const orc_dyn *d, const orc_md_state *st,
+                           double kd[9])
+{
+   double hi[9], p[3];
+   int sp, m, i, j, imol = 0;
+   orc_invert(st->h, hi);
+   for (i = 0; i < 9; i++)
+      kd[i] = 0.0;
+   for (sp = 0; sp < s->nspecies; sp++) {
+      int nm = s->spec_nmols[sp];
+      double dy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (m = imol; m < imol + nm; m++) {
+         const double *q = st->mom + 3 * m;
+         for (i = 0; i < 3; i++)
+            p[i] = hi[i] * q[0] + hi[3 + i] * q[1] + hi[6 + i] * q[2];
+         for (i = 0; i < 3; i++)
+            for (j = 0; j < 3; j++)
+               dy[3 * i + j] += p[i] * p[j];
+      }
+      for (i = 0; i < 9; i++)
+         kd[i] += dy[i] / (d->spec_mass[sp] * st->ts * st->ts);
+      imol += nm;
+   }
+}
+
+int orc_do_step(const orc_sys *s, const orc_dyn *d, orc_md_state *st, int init_H_0,
+                double pe[2], double stress[9], double rec[26])
 {
    int nm = s->nmols, sp, rc, bad = -1, imol, saxis = 0, m;
-   double step = d->step, ke_t, ke_r, dip[3];
+   const int press = d->const_pressure != 0;
+   const int uni = d->const_pressure > 0 && d->const_pressure % 2 == 0;
+   double step = d->step, ke_t, ke_r, dip[3], kd[9];
+   double *h = st->h;
    double *force = (double *)calloc((size_t)nm * 6 + 6, sizeof(double)), *torque;
    if (!force)
       return ORC_ERR_ALLOC;
@@ -1643,10 +1769,17 @@ int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_st
       orc_nose_therm(0.5 * step, d->ttmass, &st->ts, &st->tsmom);
       st->tsmom -= 0.5 * step * d->gkt * (1.0 + log(st->ts));
    }
+   /* H2 half step */
+   if (press)
+      orc_gleap_cell(d, 0.5 * step, st->ts, h, st->hmom, &st->tsmom, uni);
    /* H3 half step */
    if (d->const_temp) {
       orc_tot_ke(s, d, h, st, &ke_t, &ke_r);
       st->tsmom += 0.5 * step * ke_t;
+   }
+   if (press) {
+      orc_stress_kin(s, d, st, kd);
+      orc_update_hmom(0.5 * step, st->ts, h, kd, st->hmom, uni);
    }
    rc = orc_leapf_all_coords(s, d, h, 0.5 * step, st, saxis, &bad);
    if (rc)
@@ -1658,9 +1791,12 @@ int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_st
       goto out;
    orc_leapf_all_momenta(s, h, 0.5 * step, st, force, torque);
    orc_tot_ke(s, d, h, st, &ke_t, &ke_r);
-   if (init_H_0)
+   if (init_H_0) {
       st->H_0 = (ke_t + ke_r) + (pe[0] + pe[1]) + st->tsmom * st->tsmom / (2.0 * d->ttmass) +
-                d->gkt * log(st->ts) + 0.5 * 0.0 / d->pmass;
+                d->gkt * log(st->ts) + orc_ke_cell(st->hmom, d->pmass);
+      if (press)
+         st->H_0 += d->pressure * orc_det(h);
+   }
    if (rec) {
       rec[0] = ke_t;
       rec[1] = ke_r;
@@ -1669,6 +1805,11 @@ int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_st
       rec[4] = st->ts;
       rec[5] = st->tsmom;
       rec[6] = st->H_0;
+      rec[7] = orc_det(h);
+      for (m = 0; m < 9; m++) {
+         rec[8 + m] = st->hmom[m];
+         rec[17 + m] = h[m];
+      }
    }
    orc_leapf_all_momenta(s, h, 0.5 * step, st, force, torque);
    /* frameworks do not move, src/accel.c:752-757 */
@@ -1680,13 +1821,24 @@ int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_st
    }
    if (d->const_temp)
       st->tsmom -= step * (pe[0] + pe[1] - st->H_0);
+   if (press)
+      orc_update_hmom(step, st->ts, h, stress, st->hmom, uni);
+   /* second H3 half step */
    if (d->const_temp) {
       orc_tot_ke(s, d, h, st, &ke_t, &ke_r);
       st->tsmom += 0.5 * step * ke_t;
    }
+   if (press) {
+      orc_stress_kin(s, d, st, kd);
+      orc_update_hmom(0.5 * step, st->ts, h, kd, st->hmom, uni);
+   }
    rc = orc_leapf_all_coords(s, d, h, 0.5 * step, st, saxis, &bad);
    if (rc)
       goto out;
+   /* second H2 half step */
+   if (press)
+      orc_gleap_cell(d, 0.5 * step, st->ts, h, st->hmom, &st->tsmom, uni);
+   /* final H1 half step */
    if (d->const_temp) {
       st->tsmom -= 0.5 * step * d->gkt * (1.0 + log(st->ts));
       orc_nose_therm(0.5 * step, d->ttmass, &st->ts, &st->tsmom);

@@ -164,6 +164,9 @@ typedef struct {
    int nosymmetric_rot;          /* control.nosymmetric_rot: leapf_quat_a     */
    const double *spec_mass;      /* [nspecies]                                */
    const double *spec_inertia;   /* [nspecies][3] principal moments           */
+   int const_pressure;           /* 0; 1 Parrinello-Rahman; 2 uniform dilation */
+   double pressure;              /* control.pressure, program units           */
+   int strain_mask;              /* control.strain_mask: bit 3i+j freezes h_ij */
 } orc_dyn;
 
 typedef struct {
@@ -173,13 +176,16 @@ typedef struct {
    double *amom;                 /* [nmols_r][4] principal angular momenta    */
    double ts, tsmom, H_0;        /* thermostat s, its momentum, H_0           */
    int bad_quat;                 /* molecule of a QNORM failure               */
+   double h[9], hmom[9];         /* cell matrix and its momentum (barostat)   */
 } orc_md_state;
 
-/* One do_step.  pe[2] and stress[9] of the force evaluation; rec[7] (optional)
- * = translational KE, rotational KE, pe[0], pe[1], s, p_s and H_0 at the
- * point where do_step evaluates H_0 (after the first momentum half step). */
-int orc_do_step(const orc_sys *s, const orc_dyn *d, const double h[9], orc_md_state *st,
-                int init_H_0, double pe[2], double stress[9], double rec[7]);
+/* One do_step on st (its h is the cell; it changes only with a barostat).
+ * pe[2] and stress[9] of the force evaluation; rec[26] (optional)
+ * = translational KE, rotational KE, pe[0], pe[1], s, p_s, H_0, det(h),
+ * hmom[9] and h[9] at the point where do_step evaluates H_0 (after the first
+ * momentum half step). */
+int orc_do_step(const orc_sys *s, const orc_dyn *d, orc_md_state *st, int init_H_0,
+                double pe[2], double stress[9], double rec[26]);
 int orc_symmetry_axis(const double *inertia);
 
 /* ---- radial distribution functions: rdf_inner (src/force.c:1010-1103) and

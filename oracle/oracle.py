@@ -48,14 +48,15 @@ class EwInfo(C.Structure):
 class OrcDyn(C.Structure):
     _fields_ = [("step", C.c_double), ("ttmass", C.c_double), ("pmass", C.c_double),
                 ("gkt", C.c_double), ("const_temp", C.c_int), ("nosymmetric_rot", C.c_int),
-                ("spec_mass", C.POINTER(C.c_double)), ("spec_inertia", C.POINTER(C.c_double))]
+                ("spec_mass", C.POINTER(C.c_double)), ("spec_inertia", C.POINTER(C.c_double)),
+                ("const_pressure", C.c_int), ("pressure", C.c_double), ("strain_mask", C.c_int)]
 
 
 class OrcMdState(C.Structure):
     _fields_ = [("c_of_m", C.POINTER(C.c_double)), ("quat", C.POINTER(C.c_double)),
                 ("mom", C.POINTER(C.c_double)), ("amom", C.POINTER(C.c_double)),
                 ("ts", C.c_double), ("tsmom", C.c_double), ("H_0", C.c_double),
-                ("bad_quat", C.c_int)]
+                ("bad_quat", C.c_int), ("h", C.c_double * 9), ("hmom", C.c_double * 9)]
 
 
 def _build():
@@ -236,7 +237,10 @@ class Oracle:
 
     def md(self, dyn, mom, amom, c_of_m=None, quat=None, ts=1.0, tsmom=0.0, H_0=0.0):
         """Oracle MD run: returns a callable step(init_H_0) -> (rc, pe, stress,
-        rec) that advances its own copy of the state, and the state dict."""
+        rec) that advances its own copy of the state, and the state dict.
+        rec: KE_trans, KE_rot, pe0, pe1, s, p_s, H_0, det(h), hmom[9], h[9] at the
+        H_0 point of do_step.  The cell starts at the system's h with zero
+        momentum."""
         s = self.sys
         mass, inertia = s.species_mass_inertia()
         st = dict(c_of_m=np.array(s.c_of_m if c_of_m is None else c_of_m, np.float64),
@@ -246,16 +250,22 @@ class Oracle:
                   mass=np.ascontiguousarray(mass), inertia=np.ascontiguousarray(inertia))
         d = OrcDyn(dyn.step, dyn.ttmass, dyn.pmass,
                    s.degrees_of_freedom() * units.KB * dyn.temp, int(dyn.const_temp),
-                   int(dyn.nosymmetric_rot), _p(st["mass"]), _p(st["inertia"]))
+                   int(dyn.nosymmetric_rot), _p(st["mass"]), _p(st["inertia"]),
+                   int(dyn.const_pressure), float(dyn.pressure), int(dyn.strain_mask))
         ms = OrcMdState(_p(st["c_of_m"]), _p(st["quat"]), _p(st["mom"]), _p(st["amom"]),
                         ts, tsmom, H_0, -1)
+        for i in range(9):
+            ms.h[i] = float(np.asarray(self.h).reshape(-1)[i])
+            ms.hmom[i] = 0.0
         st["_keep"] = (d, ms)
 
         def step(init_H_0=False):
-            pe, stress, rec = np.zeros(2), np.zeros(9), np.zeros(8)
-            rc = lib().orc_do_step(C.byref(self.cs), C.byref(d), _p(self.h), C.byref(ms),
+            pe, stress, rec = np.zeros(2), np.zeros(9), np.zeros(26)
+            rc = lib().orc_do_step(C.byref(self.cs), C.byref(d), C.byref(ms),
                                    int(init_H_0), _p(pe), _p(stress), _p(rec))
             st["ts"], st["tsmom"], st["H_0"], st["bad_quat"] = ms.ts, ms.tsmom, ms.H_0, ms.bad_quat
+            st["h"] = np.array(ms.h[:]).reshape(3, 3)
+            st["hmom"] = np.array(ms.hmom[:]).reshape(3, 3)
             return rc, pe, stress.reshape(3, 3), rec
         return step, st
 

@@ -4,6 +4,9 @@ restatement, and graph replay against eager launches.
 FP64 trajectories agree with the oracle to 1e-9 relative over a few steps
 (the same arithmetic up to summation order, amplified by a short chaotic
 horizon); graph-replayed steps are bit-identical to eagerly launched ones.
+Barostat runs (const_pressure 1 and 2, with and without the thermostat)
+also match the oracle's cell h and cell momentum, across the link-cell and
+k-vector re-plans that every step's new cell brings.
 """
 import numpy as np
 import pytest
@@ -60,13 +63,13 @@ def test_md_steps_vs_oracle(case, graph):
     for i in range(nsteps):
         rc, pe, stress, rec = step(i == 0)
         assert rc == 0
-        ref_rec.append(rec[:7])
+        ref_rec.append(rec[:8])
     ref_rec = np.array(ref_rec)
     e = _engine(s, dyn, mom, amom)
     tr, pe_g, st_g = e.md_steps(nsteps, init_H_0=True, use_graph=graph)
     got = e.download_dynamics()
     errs = dict(c_of_m=rel_err(got["c_of_m"], st["c_of_m"]), mom=rel_err(got["mom"], st["mom"]),
-                trace=rel_err(tr[:, :7], ref_rec), pe=rel_err(pe_g, pe), stress=rel_err(st_g, stress))
+                trace=rel_err(tr, ref_rec), pe=rel_err(pe_g, pe), stress=rel_err(st_g, stress))
     if s.nmols_r:
         errs["quat"] = rel_err(got["quat"], st["quat"])
         errs["amom"] = rel_err(got["amom"], st["amom"])
@@ -114,3 +117,78 @@ def test_graph_rebuilt_after_state_upload():
     tr_e, _, _ = e.md_steps(4, init_H_0=True, use_graph=False)
     assert np.array_equal(tr_g, tr_e)
     e.close()
+
+
+def barostat_cases():
+    from moldy_amd import units
+    P = 100.0 / units.CONV_P
+    return {
+        "argon_nph_pr": (lambda: S.argon(), 120.0,
+                         system.Dynamics(step=0.005, pmass=100.0, const_pressure=1, pressure=P)),
+        "argon_nph_uniform": (lambda: S.argon(), 120.0,
+                              system.Dynamics(step=0.005, pmass=100.0, const_pressure=2, pressure=P)),
+        "water_npt_pr": (lambda: S.water(), 350.0,
+                         system.Dynamics(step=0.0005, temp=350.0, const_temp=1, ttmass=50.0, pmass=10.0,
+                                         const_pressure=1, pressure=P)),
+        "water_nph_quat_a": (lambda: S.water(), 350.0,
+                             system.Dynamics(step=0.0005, pmass=10.0, const_pressure=1, pressure=P,
+                                             nosymmetric_rot=True, strain_mask=0)),
+        "salt_npt_uniform": (lambda: S.salt(), 350.0,
+                             system.Dynamics(step=0.0005, temp=350.0, const_temp=1, ttmass=50.0, pmass=10.0,
+                                             const_pressure=2, pressure=P)),
+    }
+
+
+@pytest.mark.parametrize("case", list(barostat_cases()))
+def test_barostat_steps_vs_oracle(case):
+    from oracle.oracle import Oracle
+    build, T, dyn = barostat_cases()[case]
+    s = build()
+    mom, amom = s.thermal_momenta(T, seed=3)
+    nsteps = 6
+    o = Oracle(s)
+    step, st = o.md(dyn, mom, amom)
+    ref_rec = []
+    for i in range(nsteps):
+        rc, pe, stress, rec = step(i == 0)
+        assert rc == 0
+        ref_rec.append(rec[:8])
+    ref_rec = np.array(ref_rec)
+    e = _engine(s, dyn, mom, amom)
+    # two calls: the cell and its momentum carry over between runs
+    tr1, _, _ = e.md_steps(nsteps // 2, init_H_0=True)
+    tr2, pe_g, st_g = e.md_steps(nsteps - nsteps // 2)
+    tr = np.vstack([tr1, tr2])
+    got = e.download_dynamics()
+    h, hmom = e.download_cell()
+    errs = dict(c_of_m=rel_err(got["c_of_m"], st["c_of_m"]), mom=rel_err(got["mom"], st["mom"]),
+                trace=rel_err(tr, ref_rec), pe=rel_err(pe_g, pe), stress=rel_err(st_g, stress),
+                h=rel_err(h, st["h"]), hmom=rel_err(hmom, st["hmom"]))
+    if s.nmols_r:
+        errs["quat"] = rel_err(got["quat"], st["quat"])
+        errs["amom"] = rel_err(got["amom"], st["amom"])
+    errs["tsmom"] = abs(got["tsmom"] - st["tsmom"]) / max(abs(st["tsmom"]), 1.0)
+    print(case, {k: f"{v:.1e}" for k, v in errs.items()}, "dV/V", tr[-1, 7] / tr[0, 7] - 1)
+    for k, v in errs.items():
+        assert v < TOL, (k, v)
+    assert abs(tr[-1, 7] / tr[0, 7] - 1) > 1e-4          # the cell moved
+    if dyn.strain_mask == 200:
+        assert np.all(h[[1, 2, 2], [0, 0, 1]] == 0.0)
+    e.close()
+
+
+def test_barostat_mixed_precision_tracks_fp64():
+    """The mixed-precision force path under the barostat: the cell follows
+    the FP64 run to the force tolerance."""
+    build, T, dyn = barostat_cases()["water_npt_pr"]
+    s = build()
+    mom, amom = s.thermal_momenta(T, seed=3)
+    hs = []
+    for precision in (0, 1):
+        e = _engine(s, dyn, mom, amom, precision)
+        e.md_steps(8, init_H_0=True)
+        hs.append(e.download_cell())
+        e.close()
+    (h0, m0), (h1, m1) = hs
+    assert rel_err(h1, h0) < 1e-6
+    assert rel_err(m1, m0) < 1e-3

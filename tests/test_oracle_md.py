@@ -10,7 +10,12 @@ the mathematics fixes, on the CPU:
   equations and dq/dt = q (0, w) / 2, with error falling as dt^2 -- this fixes
   the direction conventions of the rotation sub-steps;
 * Nose-Poincare: (H - H_0) s is conserved to O(dt^2) (Bond, Leimkuhler and
-  Laird 1999), with H = KE + PE + p_s^2 / 2Q + g kT ln s.
+  Laird 1999), with H = KE + PE + p_s^2 / 2Q + g kT ln s;
+* barostat: with the cell terms |p_h|^2 / 2W + P det(h), the same extended
+  Hamiltonian (or plain H at constant temperature off) is conserved to
+  O(dt^2) for Parrinello-Rahman and uniform dilation, over an ideal gas,
+  free rotors and a cold crystal whose virial drives the cell; uniform
+  dilation keeps the shape of h and the strain mask freezes its entries.
 """
 import math
 
@@ -159,3 +164,110 @@ def test_nose_poincare_invariant(kind):
         assert np.ptp(ts) > 1e-3          # the thermostat does act
     assert spans[0] < 1e-3
     assert 2.5 < spans[0] / spans[1] < 6.0, spans
+
+
+# ---- barostat (do_step's const_pressure branches: gleap_cell, update_hmom,
+# stress_kin; src/accel.c:686-703, 761-779, src/leapfrog.c:415-531) ----
+
+def _hmom_kick(h, stress, s, dt, uniform):
+    """The cell-momentum increment of update_hmom for a stress times volume."""
+    if uniform:
+        return dt * s * np.trace(stress) / (h ** 2).sum() * h
+    return stress @ np.linalg.inv(h).T * (dt * s)
+
+
+def _barostat_run(s, dyn, tt, T):
+    """The extended Hamiltonian of the record point of each step.
+
+    At the record point p has half of the potential kick but p_s and the cell
+    momentum have none of theirs (accel.c:713-761): add those halves.  The
+    reference's stress carries the impulsive term of the truncated potential,
+    (distp - dist) / V on the diagonal (force.c distant-pair correction); it is
+    the volume derivative of (distp - dist) / V, which therefore joins the
+    energy."""
+    o = Oracle(s)
+    mom, amom = s.thermal_momenta(T, seed=3)
+    step, st = o.md(dyn, mom, amom)
+    cx = o.distant_const(1) - o.distant_const(0)
+    gkt = s.degrees_of_freedom() * units.KB * dyn.temp
+    mask = np.array([(dyn.strain_mask >> k) & 1 for k in range(9)]).reshape(3, 3) == 1
+    H, vol, hs = [], [], []
+    dt = dyn.step
+    for i in range(int(round(tt / dt))):
+        rc, pe, stress, rec = step(i == 0)
+        assert rc == 0
+        ke, pot, ts, tsmom, H0, V = rec[0] + rec[1], rec[2] + rec[3], rec[4], rec[5], rec[6], rec[7]
+        hm, h = rec[8:17].reshape(3, 3), rec[17:26].reshape(3, 3)
+        assert abs(np.linalg.det(h) - V) < 1e-9 * V
+        hmc = hm + 0.5 * _hmom_kick(h, stress, ts, dt, dyn.const_pressure == 2)
+        hmc[mask] = 0.0
+        Hx = ke + pot + (hmc ** 2).sum() / (2 * dyn.pmass) + dyn.pressure * V + cx / V
+        if dyn.const_temp:
+            ps = tsmom - 0.5 * dt * (pot - H0)
+            Hx = (Hx + ps ** 2 / (2 * dyn.ttmass) + gkt * np.log(ts) - H0) * ts
+        H.append(Hx)
+        vol.append(V)
+        hs.append(h)
+    return np.array(H), np.array(vol), np.array(hs), ke, st
+
+
+def _barostat_cases():
+    P_gas = 4.0 / units.CONV_P            # about N k T / V of the gas below
+    return {
+        "gas_nph": (lambda: free_molecules([[0.0, 0.0, 0.0]], [20.0], nmols=27, rdof=0),
+                    dict(temp=300.0, pmass=0.5, pressure=P_gas)),
+        "gas_npt": (lambda: free_molecules([[0.0, 0.0, 0.0]], [20.0], nmols=27, rdof=0),
+                    dict(temp=300.0, pmass=0.5, pressure=P_gas, const_temp=1, ttmass=5.0)),
+        "rotors_npt": (lambda: water_rotor(8),
+                       dict(temp=300.0, pmass=0.5, pressure=P_gas, const_temp=1, ttmass=50.0)),
+        # cold fcc argon with the cutoff between neighbour shells, so that no
+        # pair crosses it: the virial drives the cell
+        "crystal_nph": (lambda: system.argon_fcc(ncell=4, jitter=0.02, cutoff=8.6),
+                        dict(temp=10.0, pmass=500.0, pressure=50.0 / units.CONV_P)),
+    }
+
+
+@pytest.mark.parametrize("const_pressure", [1, 2])
+@pytest.mark.parametrize("case", list(_barostat_cases()))
+def test_barostat_invariant_second_order(case, const_pressure):
+    build, kw = _barostat_cases()[case]
+    spans = []
+    for dt in (0.004, 0.002) if case == "crystal_nph" else (0.002, 0.001):
+        dyn = system.Dynamics(step=dt, const_pressure=const_pressure, **kw)
+        H, vol, hs, ke, _ = _barostat_run(build(), dyn, 0.04, kw["temp"])
+        spans.append(np.ptp(H) / ke)
+        assert np.ptp(vol) / vol.mean() > 2e-4      # the cell does move
+    assert spans[0] < 5e-5, spans
+    assert 3.0 < spans[0] / spans[1] < 5.0, spans
+
+
+def test_barostat_cell_constraints():
+    """Uniform dilation keeps the shape of h; the strain mask freezes its
+    entries (default 200: the lower triangle); mask 511 freezes the cell."""
+    build, kw = _barostat_cases()["rotors_npt"]
+    s = build()
+    s.h = s.h @ np.array([[1.0, 0.05, 0.02], [0.0, 1.0, 0.03], [0.0, 0.0, 1.0]])
+    h0 = s.h.copy()
+    _, _, hs, _, st = _barostat_run(s, system.Dynamics(step=0.002, const_pressure=2, **kw), 0.02,
+                                    kw["temp"])
+    nz = h0 != 0
+    ratio = hs[-1][nz] / h0[nz]
+    assert np.ptp(ratio) < 1e-12 and abs(ratio[0] - 1.0) > 1e-4
+    assert np.all(hs[-1][~nz] == 0.0)
+    _, _, hs, _, _ = _barostat_run(s, system.Dynamics(step=0.002, const_pressure=1, **kw), 0.02,
+                                   kw["temp"])
+    assert np.all(hs[:, [1, 2, 2], [0, 0, 1]] == 0.0)
+    assert np.abs(hs[-1] - h0)[0, 1] > 1e-6           # upper triangle is free
+    mask = 200 | 1 | 16 | 256                         # diagonal frozen too
+    _, _, hs, _, _ = _barostat_run(s, system.Dynamics(step=0.002, const_pressure=1, strain_mask=mask,
+                                                      **kw), 0.02, kw["temp"])
+    assert np.all(hs[:, [0, 1, 2], [0, 1, 2]] == h0[[0, 1, 2], [0, 1, 2]])
+    # all frozen, no thermostat: the trajectory is the constant-volume one
+    # (with the thermostat the P V terms of gleap_cell shift p_s at the H_0
+    # point, so H_0 and the trajectory differ)
+    kw = dict(kw, const_temp=0)
+    a = _barostat_run(s, system.Dynamics(step=0.002, const_pressure=1, strain_mask=511, **kw), 0.02,
+                      kw["temp"])[4]
+    b = _barostat_run(s, system.Dynamics(step=0.002, **kw), 0.02, kw["temp"])[4]
+    for k in ("c_of_m", "quat", "mom", "amom"):
+        assert np.allclose(a[k], b[k], rtol=0, atol=1e-12), k
