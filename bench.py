@@ -229,7 +229,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the force "
-              "path is built for a single MI355X; see DESIGN.md for how it would shard)",
+              "path is built for a single MI355X and runs on one GPU only)",
               file=sys.stderr)
         sys.exit(2)
     if a.copy_calibration:

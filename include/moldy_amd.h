@@ -75,6 +75,11 @@ void mgpu_destroy(mgpu_engine *eng);
 const char *mgpu_last_error(const mgpu_engine *eng);
 int  mgpu_set_system(mgpu_engine *eng, const mgpu_system *sys);
 int  mgpu_set_precision(mgpu_engine *eng, int precision);
+/* Nonzero (the default): the device-resident evaluation runs the real-space
+ * pass on a second stream beside the Ewald sum (results bit-identical to the
+ * serial order).  No counterpart in the reference, whose force_calc and
+ * ewald run one after the other on the CPU. */
+int  mgpu_set_overlap(mgpu_engine *eng, int on);
 
 /* ---- the boundary, host buffers (replaces src/force.c:1108 force_calc) ----
  * site/site_force: [3][ld]; c_of_m_frac: [nmols][3] (read when molpbc);

@@ -161,7 +161,7 @@ void finish(mgpu_batch *b, int nsteps, double *trace) {
 
 // Process-wide memory of tuned shapes: (precision, nrows, lmax, nsites/4096)
 // -> (S(k) workgroups, MFMA back-projection partitions, FP64 partitions).
-std::map<std::tuple<int, int, int, int>, std::array<int, 3>> g_tuned;
+std::map<std::tuple<int, int, int, int>, std::array<int, 4>> g_tuned;
 
 }  // namespace
 
@@ -178,6 +178,7 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       e->tune_sk_blocks = it->second[0];
       e->tune_force_parts = it->second[1];
       e->tune_npart = it->second[2];
+      e->overlap = it->second[3];
    } else {
       hipEvent_t t0, t1;
       HIPCHK(hipEventCreate(&t0));
@@ -214,10 +215,11 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       } else {
          tune(e->tune_npart, {4, 8, 16, 32});
       }
+      tune(e->overlap, {1, 0});   // real space beside the Ewald sum, or after it
       e->prof_on = prof;
       (void)hipEventDestroy(t0);
       (void)hipEventDestroy(t1);
-      g_tuned[key] = {e->tune_sk_blocks, e->tune_force_parts, e->tune_npart};
+      g_tuned[key] = {e->tune_sk_blocks, e->tune_force_parts, e->tune_npart, e->overlap};
    }
    e->geom_version++;
    if (sk_blocks) *sk_blocks = e->precision == MGPU_MIXED ? e->tune_sk_blocks : 0;

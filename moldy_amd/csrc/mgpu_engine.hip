@@ -548,12 +548,44 @@ void mgpu::ensure_step_graph(mgpu_engine *e) {
 // eval_forces on the device (src/accel.c:398-617) from d_cm/d_quat: site
 // forces, molecular forces/torques and d_out[OUT_RES..]; no host
 // synchronisation, so a caller may capture it.
+//
+// With overlap on and an Ewald sum, the real-space pass (binning, pair
+// kernel, its column sums) runs on e->side while the reciprocal pass runs on
+// e->stream; the reciprocal pass joins the side stream just before it adds
+// its forces, so every site force is summed in the same order as serially.
 void mgpu::eval_forces_device(mgpu_engine *e, bool clear_flags) {
    zero_async(e, e->d_force_site.p, sizeof(double) * 3 * e->nsp);
    launch_make_sites(e, e->molpbc ? 1 : 0);
-   real_pass_device(e, clear_flags);
-   if (e->alpha > 1e-7) recip_pass_device(e);
+   if (e->overlap && e->alpha > 1e-7) {
+      if (clear_flags) zero_async(e, e->d_iflags.p, sizeof(int) * 48);
+      HIPCHK(hipEventRecord(e->ev_fork, e->stream));
+      HIPCHK(hipStreamWaitEvent(e->side, e->ev_fork, 0));
+      std::swap(e->stream, e->side);
+      std::swap(e->d_rtmp, e->d_rtmp_side);
+      try {
+         real_pass_device(e, false);
+      } catch (...) {
+         std::swap(e->stream, e->side);
+         std::swap(e->d_rtmp, e->d_rtmp_side);
+         throw;
+      }
+      std::swap(e->stream, e->side);
+      std::swap(e->d_rtmp, e->d_rtmp_side);
+      HIPCHK(hipEventRecord(e->ev_join, e->side));
+      e->join_pending = true;
+      recip_pass_device(e);
+      join_side(e);
+   } else {
+      real_pass_device(e, clear_flags);
+      if (e->alpha > 1e-7) recip_pass_device(e);
+   }
    run_glue(e);
+}
+
+void mgpu::join_side(mgpu_engine *e) {
+   if (!e->join_pending) return;
+   HIPCHK(hipStreamWaitEvent(e->stream, e->ev_join, 0));
+   e->join_pending = false;
 }
 
 
@@ -625,6 +657,9 @@ int mgpu_create(mgpu_engine **eng, int device) {
    int rc = guarded(e, [&] {
       HIPCHK(hipSetDevice(device));
       HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+      HIPCHK(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
       e->d_iflags.reserve(64);
       HIPCHK(hipMemset(e->d_iflags.p, 0, 64 * sizeof(int)));
       e->d_out.reserve(OUT_SIZE);
@@ -646,6 +681,7 @@ void mgpu_destroy(mgpu_engine *e) {
    if (!e) return;
    (void)hipSetDevice(e->device);
    if (e->stream) (void)hipStreamSynchronize(e->stream);
+   if (e->side) (void)hipStreamSynchronize(e->side);
    auto fr = [](auto &b) { b.free(); };
    fr(e->d_site_type); fr(e->d_site_mol); fr(e->d_site_fw); fr(e->d_chg_site); fr(e->d_entry_isite);
    fr(e->d_entry_num); fr(e->d_entry_ftype); fr(e->d_entry_mol); fr(e->d_potpar); fr(e->d_mol_first);
@@ -656,18 +692,27 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cell_nsites); fr(e->d_cell_site_start); fr(e->d_scan_tmp); fr(e->d_corig); fr(e->d_cpos);
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
-   fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_rtmp_side); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
    fr(e->d_adj); fr(e->d_adj_real);
    fr(e->d_pl_nab); fr(e->d_pl_pbc); fr(e->d_pl_nfnab); fr(e->d_rdf_blk); fr(e->d_rdf_hist); fr(e->d_rdf_bins);
    fr(e->d_mom); fr(e->d_amom); fr(e->d_scal); fr(e->d_trace); fr(e->d_spec_dyn);
    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
    if (e->graph) (void)hipGraphDestroy(e->graph);
    for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+   if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+   if (e->side) (void)hipStreamDestroy(e->side);
    if (e->stream) (void)hipStreamDestroy(e->stream);
    delete e;
 }
 
 const char *mgpu_last_error(const mgpu_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int mgpu_set_overlap(mgpu_engine *e, int on) {
+   if (!e) return MGPU_E_ARG;
+   e->overlap = on != 0;
+   return 0;
+}
 
 int mgpu_set_precision(mgpu_engine *e, int p) {
    if (!e || (p != MGPU_FP64 && p != MGPU_MIXED)) return MGPU_E_ARG;

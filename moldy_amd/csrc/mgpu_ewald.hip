@@ -382,6 +382,7 @@ void mgpu::run_ewald_fp64(mgpu_engine *e) {
       ProfScope ps(e, "ewald_force");
       dim3 g((e->nsites + 255) / 256, npart);
       k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
+      join_side(e);
       k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p,
                                                                   e->d_force_site.p);
    }

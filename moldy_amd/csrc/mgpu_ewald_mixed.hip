@@ -92,6 +92,47 @@ __global__ void k_mix_tables(TabArgs a, const double *__restrict__ site, const d
    }
 }
 
+// Site-major copy of Z for the back-projection operand: per site, the padded
+// l range of Zr then Zi (K2 = 2 NCOL columns), conjugate signs applied and
+// zeros outside |l| <= lmax, hi and lo parts.  A block transposes 64 sites
+// through LDS: coalesced row reads of the l-major tables, contiguous writes.
+constexpr int ZT_SITES = 64;
+constexpr int ZT_MAXK2 = 192;   // 2 * 16 * NB for NB <= 6
+
+__global__ __launch_bounds__(256) void k_mix_zt(int nsp, int lmax, int ncol, const half_t *__restrict__ Zhr,
+                                                const half_t *__restrict__ Zlr, const half_t *__restrict__ Zhi,
+                                                const half_t *__restrict__ Zli, half_t *__restrict__ ZTh,
+                                                half_t *__restrict__ ZTl) {
+   __shared__ __attribute__((aligned(16))) half_t sh[2][ZT_SITES * ZT_MAXK2];
+   const int K2 = 2 * ncol, tid = threadIdx.x, base = blockIdx.x * ZT_SITES;
+   const int ns_blk = min(ZT_SITES, nsp - base);
+   // columns beyond 2 lmax in each half stay zero
+   for (int q = tid; q < ZT_SITES * K2; q += 256) {
+      const int c = q % K2, cc = c < ncol ? c : c - ncol;
+      if (cc > 2 * lmax) sh[0][q] = sh[1][q] = (half_t)0;
+   }
+   const int si = tid & (ZT_SITES - 1);
+   if (si < ns_blk) {
+      const int i = base + si;
+      for (int al = tid >> 6; al <= lmax; al += 4) {
+         const size_t o = (size_t)al * nsp + i;
+         const half_t rh = Zhr[o], rl = Zlr[o], ih = Zhi[o], il = Zli[o];
+         half_t *h = &sh[0][si * K2], *l = &sh[1][si * K2];
+         h[lmax + al] = rh; l[lmax + al] = rl;
+         h[lmax - al] = rh; l[lmax - al] = rl;
+         h[ncol + lmax + al] = ih; l[ncol + lmax + al] = il;
+         if (al > 0) { h[ncol + lmax - al] = -ih; l[ncol + lmax - al] = -il; }
+      }
+   }
+   __syncthreads();
+   const int nchunk = ns_blk * K2 / 8;
+   const size_t ob = (size_t)base * K2;
+   for (int q = tid; q < nchunk; q += 256) {
+      *(half8 *)(ZTh + ob + 8 * (size_t)q) = *(const half8 *)&sh[0][8 * q];
+      *(half8 *)(ZTl + ob + 8 * (size_t)q) = *(const half8 *)&sh[1][8 * q];
+   }
+}
+
 // ---------------------------------------------------- forward: S = A Z
 struct SkArgs {
    int ntile, lmax, nsp;
@@ -120,7 +161,10 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
 #pragma unroll
       for (int n = 0; n < NB; n++) acc[m][n] = floatx4{0, 0, 0, 0};
    __syncthreads();
-   // A generation: one row, eight consecutive sites per thread
+   // A generation: one row, eight consecutive sites per thread.  Every
+   // table load below is unconditional (indices clamped to valid memory,
+   // unused values masked or never read), so loads in flight are never
+   // merged through register copies and stay in flight across the loop.
    const int rr = tid >> 2, sg = tid & 3;
    const int2 hk = shk[rr];
    const bool rvalid = hk.x >= 0;
@@ -132,19 +176,20 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
       const int ib = i0 + 8 * sg;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-         xr[v] = rvalid ? *(const float4 *)(a.X + xoff + ib + 2 * v) : float4{0, 0, 0, 0};
-         yr[v] = rvalid ? *(const float4 *)(a.Y + yoff + ib + 2 * v) : float4{0, 0, 0, 0};
+         xr[v] = *(const float4 *)(a.X + xoff + ib + 2 * v);
+         yr[v] = *(const float4 *)(a.Y + yoff + ib + 2 * v);
       }
    };
    auto gen_a = [&](int i0, int buf) {
       const int ib = i0 + 8 * sg;
+      const int tlo = sl.z - ib, thi = sl.w - ib;   // valid sites ib + [tlo, thi)
       half8 arh, arl, aih, ail;
 #pragma unroll
       for (int t = 0; t < 8; t++) {
          const float4 xv = xr[t >> 1], yv = yr[t >> 1];
          const float x0 = (t & 1) ? xv.z : xv.x, x1 = (t & 1) ? xv.w : xv.y;
          const float y0 = (t & 1) ? yv.z : yv.x, y1 = ysgn * ((t & 1) ? yv.w : yv.y);
-         const bool ok = rvalid && ib + t >= sl.z && ib + t < sl.w;
+         const bool ok = rvalid & (t >= tlo) & (t < thi);
          const float ar = ok ? x0 * y0 - x1 * y1 : 0.f;
          const float ai = ok ? x1 * y0 + x0 * y1 : 0.f;
          half_t h, l;
@@ -159,58 +204,73 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
       *(half8 *)&sA[buf][3][rr][8 * sg] = ail;
    };
    // B operand of this wave (Zr or Zi, hi and lo) straight from the tables:
-   // lane (g, lc) takes sites 8g..8g+7 of column n*16+lc
+   // lane (g, lc) takes sites 8g..8g+7 of column n*16+lc.  The tables hold
+   // |l|; exp(-i l c*.r) = conj, so the Zi columns with l < 0 change sign at
+   // the store.  Padding columns (|l| > lmax) read row lmax: the reduction
+   // never reads them.  Column n of the next step is loaded as soon as this
+   // step's last product with column n has issued, a whole step ahead.
    const half_t *zh = wn ? a.Zhi : a.Zhr, *zl = wn ? a.Zli : a.Zlr;
-   half8 bh[NB], bl[NB];
-   auto load_b = [&](int i0) {
+   size_t boff[NB];
 #pragma unroll
-      for (int n = 0; n < NB; n++) {
-         const int l = n * 16 + lc - a.lmax, al = l < 0 ? -l : l;
-         half8 h = half8{0, 0, 0, 0, 0, 0, 0, 0}, lo = h;
-         if (al <= a.lmax) {
-            const size_t o = (size_t)al * a.nsp + i0 + 8 * g;
-            h = *(const half8 *)(zh + o);
-            lo = *(const half8 *)(zl + o);
-            if (wn && l < 0) { h = -h; lo = -lo; }   // exp(-i l c*.r) = conj
-         }
-         bh[n] = h;
-         bl[n] = lo;
-      }
+   for (int n = 0; n < NB; n++) {
+      const int l = n * 16 + lc - a.lmax, al = l < 0 ? -l : l;
+      boff[n] = (size_t)(al < a.lmax ? al : a.lmax) * a.nsp + 8 * g;
+   }
+   half8 bh[NB], bl[NB];
+   auto load_b = [&](int n, int i0) {
+      bh[n] = *(const half8 *)(zh + boff[n] + i0);
+      bl[n] = *(const half8 *)(zl + boff[n] + i0);
    };
    int i0 = sl.x;
    if (i0 < sl.y) {
       load_xy(i0);
       gen_a(i0, 0);
-      if (i0 + KS < sl.y) load_xy(i0 + KS);
+      load_xy(i0 + KS < sl.y ? i0 + KS : i0);
+#pragma unroll
+      for (int n = 0; n < NB; n++) {
+         load_b(n, i0);   // the loop's issue order: xy, then the columns in turn
+         __builtin_amdgcn_sched_barrier(0);
+      }
    }
-   __syncthreads();
+   // LDS-only barrier: the table loads in flight stay in flight across it
+   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    for (int s = 0; i0 < sl.y; i0 += KS, s++) {
       const int buf = s & 1;
-      load_b(i0);
-      if (i0 + KS < sl.y) {
-         gen_a(i0 + KS, buf ^ 1);
-         if (i0 + 2 * KS < sl.y) load_xy(i0 + 2 * KS);
-      }
+      const bool more = i0 + KS < sl.y;
+      const int inext = more ? i0 + KS : i0;                      // in bounds
+      const int inext2 = i0 + 2 * KS < sl.y ? i0 + 2 * KS : inext;
+      half8 ah[4], al8[4];
 #pragma unroll
       for (int m = 0; m < 4; m++) {
-         const half8 ah = *(const half8 *)&sA[buf][2 * wm][m * 16 + lc][8 * g];
-         const half8 al8 = *(const half8 *)&sA[buf][2 * wm + 1][m * 16 + lc][8 * g];
-#pragma unroll
-         for (int n = 0; n < NB; n++) {
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[n], acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[n], acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8, bh[n], acc[m][n], 0, 0, 0);
-         }
+         ah[m] = *(const half8 *)&sA[buf][2 * wm][m * 16 + lc][8 * g];
+         al8[m] = *(const half8 *)&sA[buf][2 * wm + 1][m * 16 + lc][8 * g];
       }
-      __syncthreads();
+      gen_a(inext, buf ^ 1);   // after the last step: written, never read
+      load_xy(inext2);
+#pragma unroll
+      for (int n = 0; n < NB; n++) {
+#pragma unroll
+         for (int m = 0; m < 4; m++) {
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bh[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bl[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8[m], bh[n], acc[m][n], 0, 0, 0);
+         }
+         load_b(n, inext);
+         // keep each column's products and its next loads in this order;
+         // VALU, SALU and LDS writes (the A generation) may still interleave
+         __builtin_amdgcn_sched_barrier(0x0606);
+      }
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    }
    float *out = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wm * 2 + wn) * FROWS * NCOL;
 #pragma unroll
-   for (int m = 0; m < 4; m++)
+   for (int n = 0; n < NB; n++) {
+      const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
 #pragma unroll
-      for (int n = 0; n < NB; n++)
+      for (int m = 0; m < 4; m++)
 #pragma unroll
-         for (int j = 0; j < 4; j++) out[(m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = acc[m][n][j];
+         for (int j = 0; j < 4; j++) out[(m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = sgn * acc[m][n][j];
+   }
 }
 
 // --------------------- slice reduction, energy, stress and weights (FP64)
@@ -339,12 +399,13 @@ struct FArgs {
    const int2 *rowhk;       // [nrows16] (h, k)
    const float4 *rowk;      // [nrows16] (kx, ky, kzt, 0)
    const float2 *X, *Y;
-   const half_t *Zhr, *Zlr, *Zhi, *Zli;
+   const half_t *ZTh, *ZTl; // site-major Z operand [nsp][2 NCOL]
    const half_t *Wt;
    const unsigned *maxbits;
    double *force;           // [3][nsp]
    int nparts;              // row partitions (blockIdx.x % nparts)
-   float *fpart;            // [nparts][3][nsp] when nparts > 1
+   int direct;              // add into force (one partition, nothing to wait for)
+   float *fpart;            // [nparts][3][nsp] otherwise
 };
 
 template <int NB>
@@ -352,7 +413,6 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    constexpr int NCOL = 16 * NB, K2 = 2 * NCOL;
    constexpr int CH = 4 * 2 * 16 * (K2 / 8);   // half8 chunks in one weight tile
    constexpr int CPER = (CH + 511) / 512;
-   // at NB = 6 the prefetch registers would spill: load the next tile after the epilogue
    constexpr bool EARLY = NB <= 5;
    __shared__ __attribute__((aligned(16))) half_t sW[2][4][2][16][K2 + 8];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
@@ -384,28 +444,18 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
       }
    };
    wload(rt0);
-   // Z operand [site][Zr | Zi over the padded l range], held for the whole kernel
+   // Z operand [site][Zr | Zi over the padded l range], held for the whole
+   // kernel.  Sites past s1 read the last valid row: their products land
+   // only in their own output rows, which are never stored.
    half8 zh[2][NB], zl[2][NB];
 #pragma unroll
    for (int m = 0; m < 2; m++) {
-      const int i = base + 32 * wave + 16 * m + lc;
+      const int i = base + 32 * wave + 16 * m + lc, ic = i < a.s1 ? i : a.s1 - 1;
+      const size_t o = (size_t)ic * K2 + 8 * g;
 #pragma unroll
       for (int ks = 0; ks < NB; ks++) {
-#pragma unroll
-         for (int e = 0; e < 8; e++) {
-            const int kk = ks * 32 + 8 * g + e;
-            const bool im = kk >= NCOL;
-            const int l = (im ? kk - NCOL : kk) - a.lmax, al = l < 0 ? -l : l;
-            half_t h = (half_t)0, lo = (half_t)0;
-            if (al <= a.lmax && i < a.s1) {
-               const size_t o = (size_t)al * a.nsp + i;
-               h = im ? a.Zhi[o] : a.Zhr[o];
-               lo = im ? a.Zli[o] : a.Zlr[o];
-               if (im && l < 0) { h = -h; lo = -lo; }
-            }
-            zh[m][ks][e] = h;
-            zl[m][ks][e] = lo;
-         }
+         zh[m][ks] = *(const half8 *)(a.ZTh + o + ks * 32);
+         zl[m][ks] = *(const half8 *)(a.ZTl + o + ks * 32);
       }
    }
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
@@ -433,6 +483,8 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
       const int r = rt * 16 + lc;
       const int2 hk = a.rowhk[r];
       const float4 kv = a.rowk[r];
+      // next tile's weights: global latency hidden by this tile's MFMAs (at
+      // NB = 6 the registers would spill: then loaded after the epilogue)
       if (EARLY && rt + 1 < ntile) wload(rt + 1);
       floatx4 acc[2][4];
 #pragma unroll
@@ -514,7 +566,7 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
          for (int j = 0; j < 4; j++) {
             const int i = base + 32 * wave + 16 * m + 4 * g + j;
             if (i < a.s1) {
-               if (a.nparts == 1) {
+               if (a.direct) {
                   a.force[i] += (double)f[m][j][0];
                   a.force[a.nsp + i] += (double)f[m][j][1];
                   a.force[2 * a.nsp + i] += (double)f[m][j][2];
@@ -596,6 +648,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const size_t oX = take(sizeof(float2) * (e->hmax + 1) * (size_t)nsp);
    const size_t oY = take(sizeof(float2) * (e->kmax + 1) * (size_t)nsp);
    const size_t oZ = take(sizeof(half_t) * 4 * (lmax + 1) * (size_t)nsp);
+   const size_t oZT = take(sizeof(half_t) * 2 * (size_t)K2 * nsp);
    const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 4 * FROWS * NCOL);
    const size_t oWd = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
    const size_t oWn = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
@@ -609,12 +662,13 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const int nred = (nrows * NCOL + 255) / 256;
    const size_t oBp = take(sizeof(double) * 8 * (size_t)nred);
    const int nparts = std::max(1, std::min(e->tune_force_parts, nrows16 / 16));
-   const size_t oFp = take(nparts > 1 ? sizeof(float) * 3 * (size_t)nparts * nsp : 16);
+   const size_t oFp = take(sizeof(float) * 3 * (size_t)nparts * nsp);
    e->d_mix.reserve(off);
    uint8_t *B = e->d_mix.p;
    float2 *X = (float2 *)(B + oX), *Y = (float2 *)(B + oY);
    half_t *Zhr = (half_t *)(B + oZ), *Zlr = Zhr + (size_t)(lmax + 1) * nsp,
           *Zhi = Zlr + (size_t)(lmax + 1) * nsp, *Zli = Zhi + (size_t)(lmax + 1) * nsp;
+   half_t *ZTh = (half_t *)(B + oZT), *ZTl = ZTh + (size_t)K2 * nsp;
    float *part = (float *)(B + oP);
    double *Wd = (double *)(B + oWd), *Wnd = (double *)(B + oWn);
    half_t *Wt = (half_t *)(B + oWt), *Wtn = (half_t *)(B + oWtn);
@@ -653,6 +707,8 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = lmax; t.ns = ns; t.nsp = nsp; t.sZ = sZ;
       k_mix_tables<<<(nsp + 255) / 256, 256, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Zhr,
                                                              Zlr, Zhi, Zli);
+      k_mix_zt<<<(nsp + ZT_SITES - 1) / ZT_SITES, 256, 0, e->stream>>>(nsp, lmax, NCOL, Zhr, Zlr, Zhi, Zli, ZTh,
+                                                                      ZTl);
    }
    {
       ProfScope ps(e, "ewald_mfma_sk");
@@ -689,9 +745,10 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       f.c2 = (float)(2 * PI_D * e->hinv[8]);
       f.sZ = (float)sZ;
       f.rowhk = rowhk; f.rowk = rowk; f.X = X; f.Y = Y;
-      f.Zhr = Zhr; f.Zlr = Zlr; f.Zhi = Zhi; f.Zli = Zli; f.maxbits = maxbits;
+      f.ZTh = ZTh; f.ZTl = ZTl; f.maxbits = maxbits;
       f.force = e->d_force_site.p;
       f.nparts = nparts;
+      f.direct = nparts == 1 && !e->join_pending;
       f.fpart = fpart;
       const size_t shm = 0;
       auto run = [&](int s0, int s1, const half_t *wt) {
@@ -701,9 +758,11 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
 #define CALL_F(nb) launch_force<nb>(g, shm, e->stream, f)
          NB_SWITCH(NB, CALL_F)
 #undef CALL_F
-         if (nparts > 1)
+         if (!f.direct) {
+            join_side(e);
             k_mix_fsum<<<(s1 - s0 + 255) / 256, 256, 0, e->stream>>>(s0, s1, nsp, nparts, fpart,
                                                                      e->d_force_site.p);
+         }
       };
       run(0, e->nsitesxf, Wt);
       run(e->nsitesxf, ns, Wtn);

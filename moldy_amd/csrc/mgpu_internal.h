@@ -74,6 +74,12 @@ struct KRow {
 struct mgpu_engine {
    int device = 0;
    hipStream_t stream = nullptr;
+   // Real-space pass of eval_forces on its own stream, concurrent with the
+   // reciprocal pass; joined before the reciprocal forces are added.
+   hipStream_t side = nullptr;
+   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+   bool join_pending = false;
+   int overlap = 1;
    int precision = MGPU_FP64;
    std::string err;
 
@@ -114,6 +120,7 @@ struct mgpu_engine {
    mgpu::DBuf<double> d_red;                   // reduction scratch
    mgpu::DBuf<double> d_out;                   // small result block
    mgpu::DBuf<double> d_rtmp;                  // column-sum partials
+   mgpu::DBuf<double> d_rtmp_side;             // the same for the side stream
    mgpu::DBuf<int> d_iflags;                   // status flags / counters
 
    // ---- cells ----
@@ -254,6 +261,8 @@ void md_load_cell(mgpu_engine *e);
 void md_step_pressure(mgpu_engine *e, bool init_h0);
 void md_fetch_cell(mgpu_engine *e, double h[9], double hmom[9]);
 void set_geometry(mgpu_engine *e, const double h[9]);
+// Main stream waits for the side stream's real-space pass (if one is pending).
+void join_side(mgpu_engine *e);
 void ensure_step_graph(mgpu_engine *e);
 void autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart);
 void handle_flags(mgpu_engine *e, const int *fl);   // throws on error flags

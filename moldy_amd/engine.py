@@ -207,6 +207,10 @@ class ForceEngine:
         self._chk(lib().mgpu_set_precision(self.h_, int(p)))
         self.precision = p
 
+    def set_overlap(self, on):
+        """Real-space pass on a second stream beside the Ewald sum."""
+        self._chk(lib().mgpu_set_overlap(self.h_, int(bool(on))))
+
     # ------------------------------------------------------- the boundary
     def force_calc(self, h, xyz, fxyz=None, chg=None, c_of_m=None):
         n = self.sys.nsites

@@ -1,6 +1,6 @@
 """Writes tests/golden/reference_examples.json from the reference tree.
 
-Run in the build container (the reference tree is not on the GPU box):
+Run where the reference source tree is present (tests read only the fixture):
     python tests/golden/make_golden.py /root/reference/src/examples
 
 The fixture holds, for each example run that Moldy ships with its output

@@ -1,5 +1,5 @@
 """Mixed precision (FP32 pair forces, split-FP16 MFMA Ewald, FP64 reductions)
-against the FP64 path.  Tolerance from the north-star metric: RMS relative
+against the FP64 path.  The accuracy bar of the mixed path: RMS relative
 force error <= 1e-5; energies and stress are reported per quantity and held to
 1e-6 relative here."""
 import numpy as np
@@ -11,7 +11,7 @@ from test_gpu_parity import CASES
 
 pytestmark = pytest.mark.gpu
 
-TOL_F = 1e-5      # forces
+TOL_F = 1e-5      # forces: the mixed path's accuracy bar
 TOL_E = 1e-6      # reciprocal-space energy
 TOL_E_REAL = 1e-4 # real-space energy: FP32 pair energies of cancelling Coulomb terms
 TOL_S = 1e-5      # stress: FP32 pair virials of ionic crystals cancel strongly
