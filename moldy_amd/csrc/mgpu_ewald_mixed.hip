@@ -461,15 +461,14 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
    const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
    __shared__ float2 sY16[BSITES];   // exp(16 i b*.r) per site of the block
+   // exp(i k.r) of each lane's current row at its 8 sites, lane-private in
+   // LDS (registers go to the operands)
+   __shared__ float2 sAV[8][2][4][64];
    float f[2][4][3];
-   float2 av[2][4];   // exp(i k.r) of the lane's current row, per site
 #pragma unroll
    for (int m = 0; m < 2; m++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-         f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
-         av[m][j] = make_float2(0, 0);
-      }
+      for (int j = 0; j < 4; j++) f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
    if (tid < BSITES) {
       const int i = base + tid;
       sY16[tid] = (a.kmax >= 16 && i < a.s1) ? a.Y[(size_t)16 * a.nsp + i] : make_float2(0, 0);
@@ -492,18 +491,24 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
 #pragma unroll
          for (int kd = 0; kd < 4; kd++) acc[m][kd] = floatx4{0, 0, 0, 0};
 #pragma unroll
-      for (int ks = 0; ks < NB; ks++) {
+      for (int p = 0; p < 4 * NB; p++) {
+         const int ks = p >> 2, kd = p & 3;
+         const half8 bh = *(const half8 *)&sW[buf][kd][0][lc][ks * 32 + 8 * g];
+         const half8 bl = *(const half8 *)&sW[buf][kd][1][lc][ks * 32 + 8 * g];
 #pragma unroll
-         for (int kd = 0; kd < 4; kd++) {
-            const half8 bh = *(const half8 *)&sW[buf][kd][0][lc][ks * 32 + 8 * g];
-            const half8 bl = *(const half8 *)&sW[buf][kd][1][lc][ks * 32 + 8 * g];
-#pragma unroll
-            for (int m = 0; m < 2; m++) {
-               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
-               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
-               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
-            }
+         for (int m = 0; m < 2; m++) {
+            acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
+            acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
+            acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
          }
+      }
+      // issue order: the weight fragments of pair p + 1 (two LDS reads) go
+      // out before the six MFMAs of pair p, hiding the LDS latency
+      __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
+#pragma unroll
+      for (int p = 0; p < 4 * NB; p++) {
+         __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
+         if (p + 2 < 4 * NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
       }
       if (r < a.nrows) {
          const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
@@ -517,7 +522,7 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
                float2 A;
                if (step16) {
                   // exp(i (k+16) b*.r) = exp(i k b*.r) exp(i 16 b*.r)
-                  const float2 y = av[m][j], s = sY16[i - base];
+                  const float2 y = sAV[wave][m][j][lane], s = sY16[i - base];
                   A = make_float2(y.x * s.x - y.y * s.y, y.y * s.x + y.x * s.y);
                } else {
                   const bool in = i < a.s1;
@@ -526,7 +531,7 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
                   y.y *= ysg;
                   A = make_float2(x.x * y.x - x.y * y.y, x.y * y.x + x.x * y.y);
                }
-               av[m][j] = A;
+               sAV[wave][m][j][lane] = A;
                const float Ar = A.x, Ai = A.y;
                const float Pr = acc[m][0][j] * inv_pw, Pi = acc[m][1][j] * inv_pw;
                const float Qr = acc[m][2][j] * inv_qw, Qi = acc[m][3][j] * inv_qw;
