@@ -35,7 +35,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr double PI_D = 3.14159265358979323846;
 constexpr int KS = 32;        // sites per MFMA k step
 constexpr int FROWS = 64;     // complex rows per forward workgroup
-constexpr int BSITES = 256;   // sites per backward workgroup
+constexpr int BSITES = 128;   // sites per backward workgroup
 
 __device__ __forceinline__ void split2(double v, half_t &hi, half_t &lo) {
    float f = (float)v;
@@ -408,13 +408,23 @@ struct FArgs {
    float *fpart;            // [nparts][3][nsp] otherwise
 };
 
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void global_void;
+
+// Four waves of 32 sites each (two m tiles of 16), 128 sites a workgroup and
+// two workgroups per CU, so one workgroup's epilogue runs beside the other's
+// MFMAs.  The weight tile (16 rows, 4 kinds x hi/lo, K2 columns) is single-
+// buffered in LDS and filled by LDS DMA (global_load_lds_dwordx4) during the
+// epilogue: no registers stage it.
 template <int NB>
-__global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_force(FArgs a) {
    constexpr int NCOL = 16 * NB, K2 = 2 * NCOL;
-   constexpr int CH = 4 * 2 * 16 * (K2 / 8);   // half8 chunks in one weight tile
-   constexpr int CPER = (CH + 511) / 512;
-   constexpr bool EARLY = NB <= 5;
-   __shared__ __attribute__((aligned(16))) half_t sW[2][4][2][16][K2 + 8];
+   constexpr int CR = K2 / 8 + 1;      // 16-byte chunks per LDS row, the last one padding
+   constexpr int NINST = 2 * CR;       // 1 KB wave-wide DMA loads per tile (128 CR chunks)
+   __shared__ __attribute__((aligned(16))) half_t sW[4][2][16][K2 + 8];
+   __shared__ float2 sY16[BSITES];     // exp(16 i b*.r) per site of the block
+   // exp(i k.r) of each lane's current row at its 8 sites, lane-private
+   __shared__ float2 sAV[4][2][4][64];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
    // row partition p of the site block: consecutive workgroups take
    // consecutive partitions, so each XCD's L2 holds one slice of the weights
@@ -422,28 +432,27 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    const int base = a.s0 + (blockIdx.x / a.nparts) * BSITES;
    const int ntile_all = a.nrows16 / 16;
    const int rt0 = part * ntile_all / a.nparts, ntile = (part + 1) * ntile_all / a.nparts;
-   half8 wr[CPER];
-   auto wload = [&](int rt) {
+   // DMA of row tile rt: instruction j (wave-uniform) fills LDS bytes
+   // [1024 j, 1024 j + 1024) of sW, lane L the 16 bytes at 1024 j + 16 L;
+   // padding chunks take any valid source (never read)
+   // Each wave issues the same number of DMA loads (the surplus ones repeat
+   // the last load, same bytes to the same place), so the wait counts for
+   // the loads issued before them are known at compile time.
+   constexpr int NPW = (NINST + 3) / 4;
+   const int swave = __builtin_amdgcn_readfirstlane(wave);
+   auto wdma = [&](int rt) {
 #pragma unroll
-      for (int j = 0; j < CPER; j++) {
-         const int q = tid + 512 * j;
-         if (q < CH) {
-            const int c8 = q % (K2 / 8), row = (q / (K2 / 8)) % 16, kh = q / (16 * (K2 / 8));
-            wr[j] = *(const half8 *)(a.Wt + ((size_t)kh * a.nrows16 + rt * 16 + row) * K2 + 8 * c8);
-         }
+      for (int jj = 0; jj < NPW; jj++) {
+         const int j = min(swave + 4 * jj, NINST - 1);
+         const int q = 64 * j + lane;
+         const int reg = q / (16 * CR), rem = q - reg * 16 * CR, row = rem / CR, c = rem - row * CR;
+         const int cc = c < K2 / 8 ? c : 0;
+         const half_t *src = a.Wt + ((size_t)reg * a.nrows16 + rt * 16 + row) * K2 + 8 * cc;
+         __builtin_amdgcn_global_load_lds((global_void *)src, (lds_void *)((char *)&sW[0][0][0][0] + 1024 * j),
+                                          16, 0, 0);
       }
    };
-   auto wstore = [&](int buf) {
-#pragma unroll
-      for (int j = 0; j < CPER; j++) {
-         const int q = tid + 512 * j;
-         if (q < CH) {
-            const int c8 = q % (K2 / 8), row = (q / (K2 / 8)) % 16, kh = q / (16 * (K2 / 8));
-            *(half8 *)&sW[buf][kh >> 1][kh & 1][row][8 * c8] = wr[j];
-         }
-      }
-   };
-   wload(rt0);
+   wdma(rt0);
    // Z operand [site][Zr | Zi over the padded l range], held for the whole
    // kernel.  Sites past s1 read the last valid row: their products land
    // only in their own output rows, which are never stored.
@@ -460,10 +469,6 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
    }
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
    const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
-   __shared__ float2 sY16[BSITES];   // exp(16 i b*.r) per site of the block
-   // exp(i k.r) of each lane's current row at its 8 sites, lane-private in
-   // LDS (registers go to the operands)
-   __shared__ float2 sAV[8][2][4][64];
    float f[2][4][3];
 #pragma unroll
    for (int m = 0; m < 2; m++)
@@ -474,17 +479,12 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
       sY16[tid] = (a.kmax >= 16 && i < a.s1) ? a.Y[(size_t)16 * a.nsp + i] : make_float2(0, 0);
    }
    int cur_h = -1, cur_k = -1000000;
-   wstore(0);
-   __syncthreads();
+   // row metadata of the lane's row, one tile ahead of its use
+   int2 hk = a.rowhk[rt0 * 16 + lc];
+   float4 kv = a.rowk[rt0 * 16 + lc];
+   asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
    for (int rt = rt0; rt < ntile; rt++) {
-      const int buf = (rt - rt0) & 1;
-      // this tile's row metadata, in flight during the MFMAs
       const int r = rt * 16 + lc;
-      const int2 hk = a.rowhk[r];
-      const float4 kv = a.rowk[r];
-      // next tile's weights: global latency hidden by this tile's MFMAs (at
-      // NB = 6 the registers would spill: then loaded after the epilogue)
-      if (EARLY && rt + 1 < ntile) wload(rt + 1);
       floatx4 acc[2][4];
 #pragma unroll
       for (int m = 0; m < 2; m++)
@@ -493,8 +493,8 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
 #pragma unroll
       for (int p = 0; p < 4 * NB; p++) {
          const int ks = p >> 2, kd = p & 3;
-         const half8 bh = *(const half8 *)&sW[buf][kd][0][lc][ks * 32 + 8 * g];
-         const half8 bl = *(const half8 *)&sW[buf][kd][1][lc][ks * 32 + 8 * g];
+         const half8 bh = *(const half8 *)&sW[kd][0][lc][ks * 32 + 8 * g];
+         const half8 bl = *(const half8 *)&sW[kd][1][lc][ks * 32 + 8 * g];
 #pragma unroll
          for (int m = 0; m < 2; m++) {
             acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
@@ -510,29 +510,63 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
          __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
          if (p + 2 < 4 * NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
       }
+      // phase state and exp(16 i b*.r) of the lane's 8 sites, read before
+      // the DMA goes out (an LDS access after LDS DMA waits for the DMA)
+      float2 yv[2][4], sv[2][4];
+#pragma unroll
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+         for (int j = 0; j < 4; j++) {
+            yv[m][j] = sAV[wave][m][j][lane];
+            sv[m][j] = sY16[32 * wave + 16 * m + 4 * g + j];
+         }
+      // every wave is done with this tile's weights: the next tile's DMA
+      // runs under the epilogue (after the last tile it reloads this one,
+      // unread, so that every path issues the same loads)
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      const int rnext = rt + 1 < ntile ? rt + 1 : rt;
+      wdma(rnext);
+      const int2 hk_n = a.rowhk[rnext * 16 + lc];
+      const float4 kv_n = a.rowk[rnext * 16 + lc];
       if (r < a.nrows) {
          const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
-         const int ak = hk.y < 0 ? -hk.y : hk.y;
-         const float ysg = hk.y < 0 ? -1.f : 1.f;
+         if (step16) {
+            // exp(i (k+16) b*.r) = exp(i k b*.r) exp(i 16 b*.r)
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+               for (int j = 0; j < 4; j++) {
+                  const float2 y = yv[m][j], s = sv[m][j];
+                  yv[m][j] = make_float2(y.x * s.x - y.y * s.y, y.y * s.x + y.x * s.y);
+               }
+         } else {
+            // a new (h, k) run: exp(i h a*.r) exp(i k b*.r) from the tables,
+            // all loads of an m tile in flight together (sites past s1 read
+            // the last valid site; their results are never stored)
+            const int ak = hk.y < 0 ? -hk.y : hk.y;
+            const float ysg = hk.y < 0 ? -1.f : 1.f;
+            const float2 *xrow = a.X + (size_t)hk.x * a.nsp, *yrow = a.Y + (size_t)ak * a.nsp;
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+               float2 x[4], y[4];
+#pragma unroll
+               for (int j = 0; j < 4; j++) {
+                  const int i = base + 32 * wave + 16 * m + 4 * g + j, ic = i < a.s1 ? i : a.s1 - 1;
+                  x[j] = xrow[ic];
+                  y[j] = yrow[ic];
+               }
+#pragma unroll
+               for (int j = 0; j < 4; j++) {
+                  const float yi = ysg * y[j].y;
+                  yv[m][j] = make_float2(x[j].x * y[j].x - x[j].y * yi, x[j].y * y[j].x + x[j].x * yi);
+               }
+            }
+         }
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-               const int i = base + 32 * wave + 16 * m + 4 * g + j;
-               float2 A;
-               if (step16) {
-                  // exp(i (k+16) b*.r) = exp(i k b*.r) exp(i 16 b*.r)
-                  const float2 y = sAV[wave][m][j][lane], s = sY16[i - base];
-                  A = make_float2(y.x * s.x - y.y * s.y, y.y * s.x + y.x * s.y);
-               } else {
-                  const bool in = i < a.s1;
-                  const float2 x = in ? a.X[(size_t)hk.x * a.nsp + i] : make_float2(0, 0);
-                  float2 y = in ? a.Y[(size_t)ak * a.nsp + i] : make_float2(0, 0);
-                  y.y *= ysg;
-                  A = make_float2(x.x * y.x - x.y * y.y, x.y * y.x + x.x * y.y);
-               }
-               sAV[wave][m][j][lane] = A;
-               const float Ar = A.x, Ai = A.y;
+               const float Ar = yv[m][j].x, Ai = yv[m][j].y;
                const float Pr = acc[m][0][j] * inv_pw, Pi = acc[m][1][j] * inv_pw;
                const float Qr = acc[m][2][j] * inv_qw, Qi = acc[m][3][j] * inv_qw;
                const float t1 = Ar * Pi + Ai * Pr;
@@ -544,11 +578,16 @@ __global__ __launch_bounds__(512) void k_mix_force(FArgs a) {
          cur_h = hk.x;
          cur_k = hk.y;
       }
-      if (rt + 1 < ntile) {
-         if (!EARLY) wload(rt + 1);
-         wstore(buf ^ 1);
+      // the next tile's weights are in LDS for every wave
+      asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      if (r < a.nrows) {
+#pragma unroll
+         for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) sAV[wave][m][j][lane] = yv[m][j];
       }
-      __syncthreads();
+      hk = hk_n;
+      kv = kv_n;
    }
    // lanes with the same g hold the same sites: sum over the 16 row lanes
 #pragma unroll
@@ -603,7 +642,7 @@ template <int NB> void launch_sk(dim3 g, hipStream_t s, const SkArgs &a) {
    k_mix_sk<NB><<<g, 256, 0, s>>>(a);
 }
 template <int NB> void launch_force(dim3 g, size_t, hipStream_t s, const FArgs &a) {
-   k_mix_force<NB><<<g, 512, 0, s>>>(a);
+   k_mix_force<NB><<<g, 256, 0, s>>>(a);
 }
 
 #define NB_SWITCH(nb, CALL)                                                     \
