@@ -164,7 +164,7 @@ def replica_leg(nrep, steps, seed=9):
         engines.append(e)
     b = E.Batch(engines)
     b.steps(2, init_H_0=True)
-    sched, ms = b.autotune(trial_steps=4)
+    sched, ms = b.autotune(trial_steps=8)
     b.steps(2, init_H_0=True)                  # graph capture outside the timing
     t0 = time.perf_counter()
     b.steps(steps)

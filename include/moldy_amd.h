@@ -206,8 +206,9 @@ int mgpu_adjoint(mgpu_engine *eng, const double *v, const double *dtheta, double
  *   -1                        the batch's current choice (autotuned).
  * trace: [n_engines][nsteps][8] as mgpu_md_steps, or NULL.
  * mgpu_batch_autotune times every engine's launch shapes (mgpu_autotune) and
- * then each schedule over trial_steps MD steps, which advance the replicas;
- * ms_per_step[3] receives the wall time per step of each schedule.
+ * then each schedule over trial_steps MD steps, which advance the replicas,
+ * with the engines' own real/reciprocal stream overlap and with it off;
+ * ms_per_step[3] receives the best wall time per step of each schedule.
  * mgpu_autotune picks e's launch shapes by timing its force evaluation:
  * the S(k) MFMA workgroup count and the MFMA back-projection row partitions
  * (mixed; returned in sk_blocks and npart), or the FP64 back-projection row

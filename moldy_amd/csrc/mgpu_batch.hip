@@ -300,23 +300,42 @@ int mgpu_batch_autotune(mgpu_batch *b, int trial_steps, int *schedule, double ms
       }
       HIPCHK(hipSetDevice(b->device));
       for (auto *e : b->eng) autotune_engine(e, 3, nullptr, nullptr);
-      drop_graph(b);
-      double t[3] = {0, 0, 0};
-      for (int s = 0; s < 3; s++) {
-         enqueue_steps(b, 2, false, s);     // capture / warm up outside the timing
-         finish(b, 2, nullptr);
-         HIPCHK(hipDeviceSynchronize());
-         auto t0 = std::chrono::steady_clock::now();
-         enqueue_steps(b, trial_steps, false, s);
-         finish(b, trial_steps, nullptr);
-         auto t1 = std::chrono::steady_clock::now();
-         t[s] = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps;
+      // each schedule with the engines' own real/reciprocal stream overlap
+      // and with it off (replicas already run side by side)
+      std::vector<int> own;
+      for (auto *e : b->eng) own.push_back(e->overlap);
+      auto set_overlap = [&](bool off) {
+         for (size_t r = 0; r < b->eng.size(); r++) {
+            const int want = off ? 0 : own[r];
+            if (b->eng[r]->overlap != want) {
+               b->eng[r]->overlap = want;
+               b->eng[r]->geom_version++;   // captured graphs follow the stream layout
+            }
+         }
+         drop_graph(b);
+      };
+      double t[2][3];
+      for (int off = 0; off < 2; off++) {
+         set_overlap(off);
+         for (int s = 0; s < 3; s++) {
+            enqueue_steps(b, 2, false, s);     // capture / warm up outside the timing
+            finish(b, 2, nullptr);
+            HIPCHK(hipDeviceSynchronize());
+            auto t0 = std::chrono::steady_clock::now();
+            enqueue_steps(b, trial_steps, false, s);
+            finish(b, trial_steps, nullptr);
+            auto t1 = std::chrono::steady_clock::now();
+            t[off][s] = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps;
+         }
       }
-      int best = (int)(std::min_element(t, t + 3) - t);
-      b->schedule = best;
-      if (schedule) *schedule = best;
+      int best = 0;
+      for (int k = 1; k < 6; k++)
+         if (t[k / 3][k % 3] < t[best / 3][best % 3]) best = k;
+      set_overlap(best / 3);
+      b->schedule = best % 3;
+      if (schedule) *schedule = best % 3;
       if (ms_per_step)
-         for (int s = 0; s < 3; s++) ms_per_step[s] = t[s];
+         for (int s = 0; s < 3; s++) ms_per_step[s] = std::min(t[0][s], t[1][s]);
       return 0;
    });
 }

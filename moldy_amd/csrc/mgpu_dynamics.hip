@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 using namespace mgpu;
 
@@ -638,17 +639,23 @@ void mgpu::md_set_scalars(mgpu_engine *e, const double thermo[3]) {
    }
    HIPCHK(hipMemcpyAsync(e->d_scal.p, v, sizeof v, hipMemcpyHostToDevice, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
+   memcpy(e->cell_loaded, v + S_H, sizeof e->cell_loaded);
+   e->cell_valid = true;
 }
 
-// The host's h and hmom into the device slots (start of a run).
+// The host's h and hmom into the device slots (start of a run), unless the
+// slots already hold them.
 void mgpu::md_load_cell(mgpu_engine *e) {
    double v[18];
    for (int i = 0; i < 9; i++) {
       v[i] = e->h[i];
       v[9 + i] = e->hmom[i];
    }
+   if (e->cell_valid && memcmp(v, e->cell_loaded, sizeof v) == 0) return;
    HIPCHK(hipMemcpyAsync(e->d_scal.p + S_H, v, sizeof v, hipMemcpyHostToDevice, e->stream));
    HIPCHK(hipStreamSynchronize(e->stream));
+   memcpy(e->cell_loaded, v, sizeof v);
+   e->cell_valid = true;
 }
 
 void mgpu::md_fetch_cell(mgpu_engine *e, double h[9], double hmom[9]) {

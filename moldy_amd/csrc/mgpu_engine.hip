@@ -710,7 +710,10 @@ const char *mgpu_last_error(const mgpu_engine *e) { return e ? e->err.c_str() : 
 
 int mgpu_set_overlap(mgpu_engine *e, int on) {
    if (!e) return MGPU_E_ARG;
-   e->overlap = on != 0;
+   if (e->overlap != (on != 0)) {
+      e->overlap = on != 0;
+      e->geom_version++;   // captured step graphs follow the stream layout
+   }
    return 0;
 }
 
@@ -959,10 +962,16 @@ int mgpu_md_steps(mgpu_engine *e, int nsteps, int init_h0, int use_graph, double
       zero_async(e, e->d_iflags.p, sizeof(int) * 48);
       if (e->dyn_const_pressure) {
          // the cell moves: eager steps, the host re-plans after each cell update
+         e->cell_valid = false;
          for (int k = 0; k < nsteps; k++) md_step_pressure(e, k == 0 && init_h0);
          double h[9];
          md_fetch_cell(e, h, e->hmom);
          set_geometry(e, h);
+         for (int i = 0; i < 9; i++) {
+            e->cell_loaded[i] = e->h[i];
+            e->cell_loaded[9 + i] = e->hmom[i];
+         }
+         e->cell_valid = true;
       } else {
          md_step_launch(e, init_h0 != 0);
          int done = 1;

@@ -196,6 +196,8 @@ struct mgpu_engine {
    int dyn_const_pressure = 0, dyn_strain_mask = 0;
    double dyn_pressure = 0;
    double hmom[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // cell momentum between runs
+   double cell_loaded[18];                      // h, hmom as last put in the device slots
+   bool cell_valid = false;                     // ... and still there
    std::vector<double> spec_mass, spec_inertia;
 
    // ---- diagnostics ----

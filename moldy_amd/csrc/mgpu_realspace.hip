@@ -13,9 +13,10 @@
 // (src/force.c:973-997) when site_force starts at zero.
 //
 // Link cells hold about one site at the reference's subcell (r_c/5), so
-// the work is spread as one thread per (site, quarter of the shell): many
+// the work is spread as one thread per (site, part of the shell): many
 // short, independent loops over L2-resident cell-ordered positions keep
-// enough waves in flight to hide the gather latency.
+// enough waves in flight to hide the gather latency.  The number of parts
+// (4 to 64) grows as the system shrinks, so small boxes still fill the GPU.
 #include "mgpu_internal.h"
 #include "mgpu_pair.h"
 
@@ -25,6 +26,7 @@ namespace {
 
 struct RSArgs {
    int ncells, nx, ny, nz, nfull, max_id, nsp, strict_sub;
+   int lognq;          // log2 of the shell parts per site
    int skip_intra_u;   // FP32: drop minimum-image intramolecular pair energies
    double alpha, norm, cutsq, cut100sq, half_box_sq;
    double reloc[81];
@@ -59,12 +61,11 @@ template <> struct Pos<float> {
    }
 };
 
-// One thread per (site i, quarter of the full shell): the quarter takes
-// offsets o = q, q + NQ, ... (own cell in quarter 0) and walks the sites of
-// each neighbour cell.  The NQ threads of a site are adjacent lanes and
-// combine their forces with shuffles; energy and virial go to one partial
-// per wavefront.
-constexpr int NQ = 4;
+// One thread per (site i, part q of the full shell): part q takes offsets
+// o = q, q + nq, ... (own cell in part 0) and walks the sites of each
+// neighbour cell.  The nq threads of a site are adjacent lanes of one wave
+// and combine their forces with shuffles; energy and virial go to one
+// partial per wavefront.
 
 template <class R, int PT, bool COUL>
 __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
@@ -77,7 +78,8 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
 
    const int lane = threadIdx.x & 63;
    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-   const int p = t / NQ, qtr = t % NQ;
+   const int nq = 1 << a.lognq;
+   const int p = t >> a.lognq, qtr = t & (nq - 1);
    const bool have = p < a.nsites;
    const int pi = have ? p : 0;
    const int cell = a.ccell[pi];
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
    const R norm = (R)a.norm, alpha = (R)a.alpha;
    // per-cell partials in R, the running sums in FP64
    double gx = 0, gy = 0, gz = 0, v[7] = {0, 0, 0, 0, 0, 0, 0};
-   for (int o = qtr; have && o < a.nfull; o += NQ) {
+   for (int o = qtr; have && o < a.nfull; o += nq) {
       R fx = 0, fy = 0, fz = 0, ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
       const int4 d = a.off[o];
       int ex = ix + d.x, ey = iy + d.y, ez = iz + d.z;
@@ -146,9 +148,8 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
       v[1] += (double)w00; v[2] += (double)w01; v[3] += (double)w02;
       v[4] += (double)w11; v[5] += (double)w12; v[6] += (double)w22;
    }
-   // forces: the NQ quarters of a site sit in adjacent lanes
-#pragma unroll
-   for (int m = 1; m < NQ; m <<= 1) {
+   // forces: the nq parts of a site sit in adjacent lanes
+   for (int m = 1; m < nq; m <<= 1) {
       gx += __shfl_xor(gx, m, 64);
       gy += __shfl_xor(gy, m, 64);
       gz += __shfl_xor(gz, m, 64);
@@ -186,10 +187,15 @@ void launch_pt(int pt, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
 
 void mgpu::run_realspace(mgpu_engine *e) {
    int nc = e->ncells;
-   const int nthreads = e->nsites * NQ, nblk = (nthreads + 255) / 256, nwaves = nblk * 4;
+   // shell parts per site: at least 4, more until the grid holds about 16
+   // waves per CU
+   int lognq = 2;
+   while (lognq < 6 && (long)e->nsites << lognq < 16L * 64 * e->ncu) lognq++;
+   const int nthreads = e->nsites << lognq, nblk = (nthreads + 255) / 256, nwaves = nblk * 4;
    e->d_cell_part.reserve((size_t)nwaves * 8 + 8);
    RSArgs a;
    a.ncells = nc; a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.nfull = e->n_full;
+   a.lognq = lognq;
    a.max_id = e->max_id; a.nsp = e->nsp;
    a.strict_sub = e->strict && !e->molpbc;
    a.alpha = e->alpha;
