@@ -197,7 +197,7 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
          return ms;
       };
       // coordinate descent over one shape at a time; workgroup targets in
-      // whole rounds of the resident slots (two per CU)
+      // whole rounds of the resident slots
       auto tune = [&](int &field, const std::vector<int> &cand) {
          float best_ms = 1e30f;
          int best = field;
@@ -209,8 +209,8 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
          field = best;
       };
       if (e->precision == MGPU_MIXED) {
-         const int slots = 2 * e->ncu;
-         tune(e->tune_sk_blocks, {slots, 2 * slots, 3 * slots, 4 * slots, 6 * slots});
+         const int slots = e->ncu;   // one S(k) workgroup per CU (LDS)
+         tune(e->tune_sk_blocks, {2 * slots, 3 * slots, 4 * slots, 6 * slots, 8 * slots, 12 * slots});
          tune(e->tune_force_parts, {1, 2, 4, 8, 16});
       } else {
          tune(e->tune_npart, {4, 8, 16, 32});

@@ -666,7 +666,7 @@ int mgpu_create(mgpu_engine **eng, int device) {
       hipDeviceProp_t prop;
       HIPCHK(hipGetDeviceProperties(&prop, device));
       e->ncu = std::max(1, prop.multiProcessorCount);
-      e->tune_sk_blocks = 4 * e->ncu;   // two rounds of two resident workgroups per CU
+      e->tune_sk_blocks = 8 * e->ncu;   // eight rounds of one resident S(k) workgroup per CU
       return 0;
    });
    if (rc) {

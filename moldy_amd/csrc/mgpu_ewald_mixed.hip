@@ -34,7 +34,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr double PI_D = 3.14159265358979323846;
 constexpr int KS = 32;        // sites per MFMA k step
-constexpr int FROWS = 64;     // complex rows per forward workgroup
+constexpr int FROWS = 128;    // complex rows per forward workgroup
 constexpr int BSITES = 128;   // sites per backward workgroup
 
 __device__ __forceinline__ void split2(double v, half_t &hi, half_t &lo) {
@@ -136,23 +136,31 @@ __global__ __launch_bounds__(256) void k_mix_zt(int nsp, int lmax, int ncol, con
 // ---------------------------------------------------- forward: S = A Z
 struct SkArgs {
    int ntile, lmax, nsp;
-   const int2 *rowhk;     // [ntile*64] (h, k); h < 0 marks padding
+   const int2 *rowhk;     // [ntile*FROWS] (h, k); h < 0 marks padding
    const float2 *X, *Y;
    const half_t *Zhr, *Zlr, *Zhi, *Zli;
    const int4 *slices;    // (i0 aligned to 32, i1, valid lo, valid hi)
-   float *part;           // [nslices][ntile][4][64][16 NB]
+   float *part;           // [nslices][ntile][4][FROWS][16 NB]
 };
 
+// Eight waves over a tile of FROWS = 128 rows: wave w takes the A part
+// wm = (w >> 1) & 1 (Ar or Ai), the Z part wn = w & 1 (Zr or Zi) and the row
+// half rh = w >> 2, so both row halves share every Z fragment.  Per 32-site
+// step: A for the next step is generated into a double-buffered LDS tile;
+// the next step's Z tile (Zr/Zi x hi/lo x 16 NB columns x 32 sites) is
+// loaded once per workgroup into registers early in the step and written
+// to its LDS buffer after the MFMAs; one barrier per step, on LDS only.
 template <int NB>
-__global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
+__global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
    constexpr int NCOL = 16 * NB;
-   // A operand double-buffered in LDS: step s+1 is generated while step s
-   // is multiplied, one barrier per step
-   __shared__ __attribute__((aligned(16))) half_t sA[2][4][FROWS][KS + 8];   // Ar hi/lo, Ai hi/lo
+   constexpr int BCH = 16 * NCOL;              // 16-byte Z chunks per step
+   constexpr int BPT = (BCH + 511) / 512;      // per thread
+   __shared__ __attribute__((aligned(16))) half_t sA[2][4][FROWS][KS + 8];       // Ar hi/lo, Ai hi/lo
+   __shared__ __attribute__((aligned(16))) half_t sB[2][2][2][NCOL][KS + 8];     // [buf][Zr|Zi][hi|lo][col][site]
    __shared__ int2 shk[FROWS];
    const int tile = blockIdx.x, slice = blockIdx.y;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-   const int wm = wave >> 1, wn = wave & 1, g = lane >> 4, lc = lane & 15;
+   const int wm = (wave >> 1) & 1, wn = wave & 1, rh = wave >> 2, g = lane >> 4, lc = lane & 15;
    if (tid < FROWS) shk[tid] = a.rowhk[tile * FROWS + tid];
    const int4 sl = a.slices[slice];
    floatx4 acc[4][NB];
@@ -161,10 +169,10 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
 #pragma unroll
       for (int n = 0; n < NB; n++) acc[m][n] = floatx4{0, 0, 0, 0};
    __syncthreads();
-   // A generation: one row, eight consecutive sites per thread.  Every
-   // table load below is unconditional (indices clamped to valid memory,
-   // unused values masked or never read), so loads in flight are never
-   // merged through register copies and stay in flight across the loop.
+   // A generation: one row, eight consecutive sites per thread.  Table
+   // loads are unconditional (indices clamped to valid memory, unused values
+   // masked or never read), so loads in flight are never merged through
+   // register copies and stay in flight across the loop.
    const int rr = tid >> 2, sg = tid & 3;
    const int2 hk = shk[rr];
    const bool rvalid = hk.x >= 0;
@@ -203,66 +211,94 @@ __global__ __launch_bounds__(256) void k_mix_sk(SkArgs a) {
       *(half8 *)&sA[buf][2][rr][8 * sg] = aih;
       *(half8 *)&sA[buf][3][rr][8 * sg] = ail;
    };
-   // B operand of this wave (Zr or Zi, hi and lo) straight from the tables:
-   // lane (g, lc) takes sites 8g..8g+7 of column n*16+lc.  The tables hold
-   // |l|; exp(-i l c*.r) = conj, so the Zi columns with l < 0 change sign at
-   // the store.  Padding columns (|l| > lmax) read row lmax: the reduction
-   // never reads them.  Column n of the next step is loaded as soon as this
-   // step's last product with column n has issued, a whole step ahead.
-   const half_t *zh = wn ? a.Zhi : a.Zhr, *zl = wn ? a.Zli : a.Zlr;
-   size_t boff[NB];
+   // Z tile staging: chunk q = (part, hi/lo, column, 8 sites c).  The tables
+   // hold |l|; exp(-i l c*.r) = conj, so the Zi columns with l < 0 change
+   // sign at the store.  Padding columns (|l| > lmax) read row lmax: the
+   // reduction never reads them.
+   half8 bst[BPT];
+   auto load_b = [&](int i0) {
 #pragma unroll
-   for (int n = 0; n < NB; n++) {
-      const int l = n * 16 + lc - a.lmax, al = l < 0 ? -l : l;
-      boff[n] = (size_t)(al < a.lmax ? al : a.lmax) * a.nsp + 8 * g;
-   }
-   half8 bh[NB], bl[NB];
-   auto load_b = [&](int n, int i0) {
-      bh[n] = *(const half8 *)(zh + boff[n] + i0);
-      bl[n] = *(const half8 *)(zl + boff[n] + i0);
+      for (int j = 0; j < BPT; j++) {
+         const int q = tid + 512 * j < BCH ? tid + 512 * j : BCH - 1;
+         const int c = q & 3, col = (q >> 2) % NCOL, hl = (q / (4 * NCOL)) & 1, part = q / (8 * NCOL);
+         const int l = col - a.lmax, al = l < 0 ? -l : l;
+         const half_t *tab = part ? (hl ? a.Zli : a.Zhi) : (hl ? a.Zlr : a.Zhr);
+         bst[j] = *(const half8 *)(tab + (size_t)(al < a.lmax ? al : a.lmax) * a.nsp + i0 + 8 * c);
+      }
+   };
+   auto store_b = [&](int buf) {
+#pragma unroll
+      for (int j = 0; j < BPT; j++) {
+         const int q = tid + 512 * j;
+         if (q < BCH) {
+            const int c = q & 3, col = (q >> 2) % NCOL, hl = (q / (4 * NCOL)) & 1, part = q / (8 * NCOL);
+            *(half8 *)&sB[buf][part][hl][col][8 * c] = bst[j];
+         }
+      }
    };
    int i0 = sl.x;
    if (i0 < sl.y) {
+      load_b(i0);
       load_xy(i0);
       gen_a(i0, 0);
+      store_b(0);
       load_xy(i0 + KS < sl.y ? i0 + KS : i0);
-#pragma unroll
-      for (int n = 0; n < NB; n++) {
-         load_b(n, i0);   // the loop's issue order: xy, then the columns in turn
-         __builtin_amdgcn_sched_barrier(0);
-      }
    }
-   // LDS-only barrier: the table loads in flight stay in flight across it
-   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+   __syncthreads();
    for (int s = 0; i0 < sl.y; i0 += KS, s++) {
       const int buf = s & 1;
-      const bool more = i0 + KS < sl.y;
-      const int inext = more ? i0 + KS : i0;                      // in bounds
+      const int inext = i0 + KS < sl.y ? i0 + KS : i0;   // in bounds
       const int inext2 = i0 + 2 * KS < sl.y ? i0 + 2 * KS : inext;
       half8 ah[4], al8[4];
 #pragma unroll
       for (int m = 0; m < 4; m++) {
-         ah[m] = *(const half8 *)&sA[buf][2 * wm][m * 16 + lc][8 * g];
-         al8[m] = *(const half8 *)&sA[buf][2 * wm + 1][m * 16 + lc][8 * g];
+         ah[m] = *(const half8 *)&sA[buf][2 * wm][64 * rh + m * 16 + lc][8 * g];
+         al8[m] = *(const half8 *)&sA[buf][2 * wm + 1][64 * rh + m * 16 + lc][8 * g];
       }
+      // table loads first: the next step's Z tile, then the X/Y rows two
+      // steps ahead (the A generation below still reads the previous ones)
+      float4 xn[4], yn[4];
+      {
+         load_b(inext);
+         const int ib = inext2 + 8 * sg;
+#pragma unroll
+         for (int v = 0; v < 4; v++) {
+            xn[v] = *(const float4 *)(a.X + xoff + ib + 2 * v);
+            yn[v] = *(const float4 *)(a.Y + yoff + ib + 2 * v);
+         }
+      }
+      // loads stay above; the A generation (VALU, LDS writes) may interleave
+      // with the MFMAs below
+      __builtin_amdgcn_sched_barrier(0x0606);
       gen_a(inext, buf ^ 1);   // after the last step: written, never read
-      load_xy(inext2);
+#pragma unroll
+      for (int v = 0; v < 4; v++) { xr[v] = xn[v]; yr[v] = yn[v]; }
 #pragma unroll
       for (int n = 0; n < NB; n++) {
+         const half8 bh = *(const half8 *)&sB[buf][wn][0][n * 16 + lc][8 * g];
+         const half8 bl = *(const half8 *)&sB[buf][wn][1][n * 16 + lc][8 * g];
 #pragma unroll
          for (int m = 0; m < 4; m++) {
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bh[n], acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bl[n], acc[m][n], 0, 0, 0);
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8[m], bh[n], acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bh, acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bl, acc[m][n], 0, 0, 0);
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al8[m], bh, acc[m][n], 0, 0, 0);
          }
-         load_b(n, inext);
-         // keep each column's products and its next loads in this order;
-         // VALU, SALU and LDS writes (the A generation) may still interleave
-         __builtin_amdgcn_sched_barrier(0x0606);
       }
+      // issue order: column n + 1's two LDS reads before column n's twelve
+      // MFMAs
+      __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
+#pragma unroll
+      for (int n = 0; n < NB; n++) {
+         __builtin_amdgcn_sched_group_barrier(0x0008, 12, 0);
+         if (n + 2 < NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
+      }
+      // the Z tile goes to LDS after the MFMAs, its loads a step old
+      __builtin_amdgcn_sched_barrier(0);
+      store_b(buf ^ 1);
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    }
-   float *out = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wm * 2 + wn) * FROWS * NCOL;
+   float *out = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wm * 2 + wn) * FROWS * NCOL +
+                (size_t)64 * rh * NCOL;
 #pragma unroll
    for (int n = 0; n < NB; n++) {
       const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
@@ -639,7 +675,7 @@ __global__ void k_mix_fsum(int s0, int s1, int nsp, int nparts, const float *__r
 
 
 template <int NB> void launch_sk(dim3 g, hipStream_t s, const SkArgs &a) {
-   k_mix_sk<NB><<<g, 256, 0, s>>>(a);
+   k_mix_sk<NB><<<g, 512, 0, s>>>(a);
 }
 template <int NB> void launch_force(dim3 g, size_t, hipStream_t s, const FArgs &a) {
    k_mix_force<NB><<<g, 256, 0, s>>>(a);
@@ -667,7 +703,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const int NCOL = 16 * NB, K2 = 2 * NCOL;
    const int nrows = e->nrows;
    const int ntile = (nrows + FROWS - 1) / FROWS;
-   const int nrows64 = ntile * FROWS, nrows16 = nrows64;
+   const int nrows64 = ntile * FROWS, nrows16 = (nrows + 15) / 16 * 16;
    // site slices: 32-aligned starts, never mixing framework and other sites
    std::vector<int4> slices;
    std::vector<int> slice_fw;
