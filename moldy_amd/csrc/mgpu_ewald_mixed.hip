@@ -140,7 +140,7 @@ struct SkArgs {
    const float2 *X, *Y;
    const half_t *Zhr, *Zlr, *Zhi, *Zli;
    const int4 *slices;    // (i0 aligned to 32, i1, valid lo, valid hi)
-   float *part;           // [nslices][ntile][4][FROWS][16 NB]
+   float *part;           // [nslices][ntile][Re|Im][FROWS][16 NB]
 };
 
 // Eight waves over a tile of FROWS = 128 rows: wave w takes the A part
@@ -297,15 +297,44 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
       store_b(buf ^ 1);
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    }
-   float *out = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wm * 2 + wn) * FROWS * NCOL +
-                (size_t)64 * rh * NCOL;
+   // Re S = Ar Zr - Ai Zi and Im S = Ar Zi + Ai Zr: the Ai waves hand their
+   // products to the Ar waves through LDS (the A tile is free after the
+   // last barrier), one row half at a time, and the Ar waves store the two
+   // parts.  Zi columns with l < 0 take the conjugate sign here.
+   float *xch = (float *)&sA[0][0][0][0];   // [wn][64 rows][NCOL]
+   static_assert(sizeof(sA) >= sizeof(float) * 2 * 64 * NCOL, "exchange tile");
 #pragma unroll
-   for (int n = 0; n < NB; n++) {
-      const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
+   for (int half = 0; half < 2; half++) {
+      if (rh == half && wm == 1) {
 #pragma unroll
-      for (int m = 0; m < 4; m++)
+         for (int n = 0; n < NB; n++) {
+            const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
 #pragma unroll
-         for (int j = 0; j < 4; j++) out[(m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = sgn * acc[m][n][j];
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+               for (int j = 0; j < 4; j++)
+                  xch[(wn * 64 + m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = sgn * acc[m][n][j];
+         }
+      }
+      __syncthreads();
+      if (rh == half && wm == 0) {
+         // (Ar, Zr) pairs with (Ai, Zi) for Re; (Ar, Zi) with (Ai, Zr) for Im
+         float *out = a.part + ((size_t)(slice * a.ntile + tile) * 2 + wn) * FROWS * NCOL + (size_t)64 * rh * NCOL;
+         const float *oth = xch + (size_t)(wn ^ 1) * 64 * NCOL;
+#pragma unroll
+         for (int n = 0; n < NB; n++) {
+            const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+               for (int j = 0; j < 4; j++) {
+                  const int o = (m * 16 + 4 * g + j) * NCOL + n * 16 + lc;
+                  const float mine = sgn * acc[m][n][j];
+                  out[o] = wn ? mine + oth[o] : mine - oth[o];
+               }
+         }
+      }
+      __syncthreads();
    }
 }
 
@@ -335,12 +364,29 @@ __global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
          const int tile = r / FROWS, rr = r - tile * FROWS;
          double scn = 0, ssn = 0, scf = 0, ssf = 0;
          const size_t pl = (size_t)FROWS * a.ncol;
-         for (int s = 0; s < a.nslices; s++) {
-            const float *p = a.part + ((size_t)(s * a.ntile + tile) * 4) * pl + rr * a.ncol + c;
-            const double sr = (double)p[0] - (double)p[3 * pl];
-            const double si = (double)p[pl] + (double)p[2 * pl];
-            if (a.slice_fw[s]) { scf += sr; ssf += si; }
-            else { scn += sr; ssn += si; }
+         // slices in order; eight slices' loads in flight at a time
+         auto add = [&](float pr, float pi, int fw) {
+            if (fw) { scf += (double)pr; ssf += (double)pi; }
+            else { scn += (double)pr; ssn += (double)pi; }
+         };
+         const float *pb = a.part + (size_t)tile * 2 * pl + rr * a.ncol + c;
+         const size_t sstride = (size_t)a.ntile * 2 * pl;
+         int s = 0;
+         for (; s + 8 <= a.nslices; s += 8) {
+            float v[8][2];
+            int fw[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+               const float *p = pb + (size_t)(s + u) * sstride;
+               v[u][0] = p[0]; v[u][1] = p[pl];
+               fw[u] = a.slice_fw[s + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) add(v[u][0], v[u][1], fw[u]);
+         }
+         for (; s < a.nslices; s++) {
+            const float *p = pb + (size_t)s * sstride;
+            add(p[0], p[pl], a.slice_fw[s]);
          }
          scn *= a.inv_sZ; ssn *= a.inv_sZ; scf *= a.inv_sZ; ssf *= a.inv_sZ;
          const double kx = R.kx, ky = R.ky, kz = R.kzt + l * a.c2;
@@ -505,6 +551,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
    }
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
    const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
+   const float c2q = a.c2 * (inv_qw / inv_pw);
    float f[2][4][3];
 #pragma unroll
    for (int m = 0; m < 2; m++)
@@ -598,18 +645,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                }
             }
          }
+         // P and Q carry power-of-two scales: the sums run in units of the
+         // P scale (exact, so the same bits as scaling every term) and the Q
+         // terms join through c2 times the ratio of the two scales
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                const float Ar = yv[m][j].x, Ai = yv[m][j].y;
-               const float Pr = acc[m][0][j] * inv_pw, Pi = acc[m][1][j] * inv_pw;
-               const float Qr = acc[m][2][j] * inv_qw, Qi = acc[m][3][j] * inv_qw;
-               const float t1 = Ar * Pi + Ai * Pr;
-               const float t2 = Ar * Qi + Ai * Qr;
+               const float t1 = Ar * acc[m][1][j] + Ai * acc[m][0][j];
+               const float t2 = Ar * acc[m][3][j] + Ai * acc[m][2][j];
                f[m][j][0] += kv.x * t1;
                f[m][j][1] += kv.y * t1;
-               f[m][j][2] += kv.z * t1 + a.c2 * t2;
+               f[m][j][2] += kv.z * t1 + c2q * t2;
             }
          cur_h = hk.x;
          cur_k = hk.y;
@@ -637,7 +685,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             v += __shfl_xor(v, 2, 64);
             v += __shfl_xor(v, 4, 64);
             v += __shfl_xor(v, 8, 64);
-            f[m][j][c] = v;
+            f[m][j][c] = v * inv_pw;
          }
    if (lc == 0) {
 #pragma unroll
@@ -729,7 +777,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const size_t oY = take(sizeof(float2) * (e->kmax + 1) * (size_t)nsp);
    const size_t oZ = take(sizeof(half_t) * 4 * (lmax + 1) * (size_t)nsp);
    const size_t oZT = take(sizeof(half_t) * 2 * (size_t)K2 * nsp);
-   const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 4 * FROWS * NCOL);
+   const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 2 * FROWS * NCOL);
    const size_t oWd = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
    const size_t oWn = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
    const size_t oWt = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
