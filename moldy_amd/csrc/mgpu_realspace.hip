@@ -92,15 +92,14 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
    const R norm = (R)a.norm, alpha = (R)a.alpha;
    // per-cell partials in R, the running sums in FP64
    double gx = 0, gy = 0, gz = 0, v[7] = {0, 0, 0, 0, 0, 0, 0};
-   for (int o = qtr; have && o < a.nfull; o += nq) {
-      R fx = 0, fy = 0, fz = 0, ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
+   // neighbour cell of offset o: index and periodic shift
+   auto cell_of = [&](int o, int &jc, R &shx, R &shy, R &shz) {
       const int4 d = a.off[o];
       int ex = ix + d.x, ey = iy + d.y, ez = iz + d.z;
       int sx = ex < 0 ? -1 : (ex >= a.nx ? 1 : 0);
       int sy = ey < 0 ? -1 : (ey >= a.ny ? 1 : 0);
       int sz = ez < 0 ? -1 : (ez >= a.nz ? 1 : 0);
-      const int jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
-      R shx, shy, shz;
+      jc = (ez - sz * a.nz) + a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
       if (sizeof(R) == 8) {
          const int code = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
          shx = (R)sreloc[3 * code]; shy = (R)sreloc[3 * code + 1]; shz = (R)sreloc[3 * code + 2];
@@ -108,14 +107,36 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
          const float4 D = a.offD[o];
          shx = D.x; shy = D.y; shz = D.z;
       }
-      const int j1 = a.css[jc + 1];
-      for (int j = a.css[jc]; j < j1; j++) {
-         if (o == 0 && j == p) continue;
+   };
+   // software pipeline over offsets: the next offset's cell and site range
+   // are fetched while this one's sites are walked
+   int jcn = 0, jn0 = 0, jn1 = 0;
+   R snx = 0, sny = 0, snz = 0;
+   if (have && qtr < a.nfull) {
+      cell_of(qtr, jcn, snx, sny, snz);
+      jn0 = a.css[jcn];
+      jn1 = a.css[jcn + 1];
+   }
+   for (int o = qtr; have && o < a.nfull; o += nq) {
+      R fx = 0, fy = 0, fz = 0, ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
+      const int j0 = jn0, j1 = jn1;
+      const R shx = snx, shy = sny, shz = snz;
+      if (o + nq < a.nfull) {
+         cell_of(o + nq, jcn, snx, sny, snz);
+         jn0 = a.css[jcn];
+         jn1 = a.css[jcn + 1];
+      }
+      for (int j = j0; j < j1; j++) {
+         // the site's metadata and position are fetched together
          const int2 mj = a.cmeta[j];
-         const int tj = mj.x & 0xff, fwj = mj.x >> 8;
-         if (fwi & fwj) continue;
          R xj, yj, zj, qj;
          Pos<R>::load(a, j, xj, yj, zj, qj);
+         // one wait for both loads (else the position load sinks below the
+         // checks and waits on its own)
+         asm volatile("" ::"v"(xj), "v"(yj), "v"(zj), "v"(qj), "v"(mj.x));
+         if (o == 0 && j == p) continue;
+         const int tj = mj.x & 0xff, fwj = mj.x >> 8;
+         if (fwi & fwj) continue;
          R rx = xj - xi + shx, ry = yj - yi + shy, rz = zj - zi + shz;
          R rsq = rx * rx + ry * ry + rz * rz;
          if (rsq < R(0.25) && mi != mj.y) {
