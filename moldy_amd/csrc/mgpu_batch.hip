@@ -198,15 +198,16 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       };
       // coordinate descent over one shape at a time; workgroup targets in
       // whole rounds of the resident slots
+      // two passes over the candidates, each scored by its faster pass, so
+      // that a clock ramp or a neighbour's burst does not pick the shape
       auto tune = [&](int &field, const std::vector<int> &cand) {
-         float best_ms = 1e30f;
-         int best = field;
-         for (int c : cand) {
-            field = c;
-            float ms = time_eval();
-            if (ms < best_ms) { best_ms = ms; best = c; }
-         }
-         field = best;
+         std::vector<float> t(cand.size(), 1e30f);
+         for (int pass = 0; pass < 2; pass++)
+            for (size_t c = 0; c < cand.size(); c++) {
+               field = cand[c];
+               t[c] = std::min(t[c], time_eval());
+            }
+         field = cand[std::min_element(t.begin(), t.end()) - t.begin()];
       };
       if (e->precision == MGPU_MIXED) {
          const int slots = e->ncu;   // one S(k) workgroup per CU (LDS)

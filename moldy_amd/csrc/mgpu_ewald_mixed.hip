@@ -50,17 +50,35 @@ __device__ __forceinline__ void split2f(float v, half_t &hi, half_t &lo) {
 // ------------------------------------------------------------- tables
 struct TabArgs {
    double ks[9];
-   int hmax, kmax, lmax, ns, nsp;
+   int hmax, kmax, lmax, ns, nsp, ncol;
    double sZ;
 };
 
 // exp(i n k*.r) by the reference's recurrence (ewald.c:148-193), FP64, then
-// FP32 for the x and y axes and split FP16 for q exp(i l c*.r).
-__global__ void k_mix_tables(TabArgs a, const double *__restrict__ site, const double *__restrict__ chg,
-                             float2 *__restrict__ X, float2 *__restrict__ Y, half_t *__restrict__ Zhr,
-                             half_t *__restrict__ Zlr, half_t *__restrict__ Zhi, half_t *__restrict__ Zli) {
-   int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i >= a.nsp) return;
+// FP32 for the x and y axes and split FP16 for q exp(i l c*.r): l-major
+// tables X[n][site], Y[n][site], Z[n][site] (the forward operands) and a
+// site-major copy of Z for the back-projection operand -- per site the
+// padded l range of Zr then Zi (K2 = 2 NCOL columns), conjugate signs
+// applied, zeros outside |l| <= lmax, hi and lo parts -- assembled in LDS
+// and written as contiguous rows.  One wave of 64 sites per block.
+constexpr int ZT_SITES = 64;
+constexpr int ZT_MAXK2 = 192;   // 2 * 16 * NB for NB <= 6
+
+__global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__restrict__ site,
+                                                   const double *__restrict__ chg, float2 *__restrict__ X,
+                                                   float2 *__restrict__ Y, half_t *__restrict__ Zhr,
+                                                   half_t *__restrict__ Zlr, half_t *__restrict__ Zhi,
+                                                   half_t *__restrict__ Zli, half_t *__restrict__ ZTh,
+                                                   half_t *__restrict__ ZTl) {
+   // rows padded by 8 halves: 4-way instead of 16-way bank conflicts
+   __shared__ __attribute__((aligned(16))) half_t sh[2][ZT_SITES * (ZT_MAXK2 + 8)];
+   const int ncol = a.ncol, K2 = 2 * ncol, RS = K2 + 8;
+   const int si = threadIdx.x, base = blockIdx.x * ZT_SITES, i = base + si;   // nsp is a multiple of 64
+   half_t *rh = &sh[0][si * RS], *rl = &sh[1][si * RS];
+   for (int c = 0; c < K2; c++) {
+      const int cc = c < ncol ? c : c - ncol;
+      if (i >= a.ns || cc > 2 * a.lmax) rh[c] = rl[c] = (half_t)0;
+   }
    if (i >= a.ns) {
       for (int n = 0; n <= a.hmax; n++) X[(size_t)n * a.nsp + i] = make_float2(0, 0);
       for (int n = 0; n <= a.kmax; n++) Y[(size_t)n * a.nsp + i] = make_float2(0, 0);
@@ -68,68 +86,42 @@ __global__ void k_mix_tables(TabArgs a, const double *__restrict__ site, const d
          size_t o = (size_t)n * a.nsp + i;
          Zhr[o] = Zlr[o] = Zhi[o] = Zli[o] = (half_t)0;
       }
-      return;
-   }
-   double x = site[i], y = site[a.nsp + i], z = site[2 * a.nsp + i], q = chg[i] * a.sZ;
-   for (int ax = 0; ax < 3; ax++) {
-      int nmax = ax == 0 ? a.hmax : (ax == 1 ? a.kmax : a.lmax);
-      double kr = a.ks[3 * ax] * x + a.ks[3 * ax + 1] * y + a.ks[3 * ax + 2] * z;
-      double s1, c1;
-      sincos(kr, &s1, &c1);
-      double cn = 1.0, sn = 0.0;
-      for (int n = 0; n <= nmax; n++) {
-         size_t o = (size_t)n * a.nsp + i;
-         if (ax == 0) X[o] = make_float2((float)cn, (float)sn);
-         else if (ax == 1) Y[o] = make_float2((float)cn, (float)sn);
-         else {
-            split2(q * cn, Zhr[o], Zlr[o]);
-            split2(q * sn, Zhi[o], Zli[o]);
+   } else {
+      const int lm = a.lmax;
+      double x = site[i], y = site[a.nsp + i], z = site[2 * a.nsp + i], q = chg[i] * a.sZ;
+      for (int ax = 0; ax < 3; ax++) {
+         int nmax = ax == 0 ? a.hmax : (ax == 1 ? a.kmax : a.lmax);
+         double kr = a.ks[3 * ax] * x + a.ks[3 * ax + 1] * y + a.ks[3 * ax + 2] * z;
+         double s1, c1;
+         sincos(kr, &s1, &c1);
+         double cn = 1.0, sn = 0.0;
+         for (int n = 0; n <= nmax; n++) {
+            size_t o = (size_t)n * a.nsp + i;
+            if (ax == 0) X[o] = make_float2((float)cn, (float)sn);
+            else if (ax == 1) Y[o] = make_float2((float)cn, (float)sn);
+            else {
+               half_t hr, lr, hi, li;
+               split2(q * cn, hr, lr);
+               split2(q * sn, hi, li);
+               Zhr[o] = hr; Zlr[o] = lr; Zhi[o] = hi; Zli[o] = li;
+               rh[lm + n] = hr; rl[lm + n] = lr;
+               rh[lm - n] = hr; rl[lm - n] = lr;
+               rh[ncol + lm + n] = hi; rl[ncol + lm + n] = li;
+               if (n > 0) { rh[ncol + lm - n] = -hi; rl[ncol + lm - n] = -li; }
+            }
+            double cc = cn * c1 - sn * s1;
+            sn = sn * c1 + cn * s1;
+            cn = cc;
          }
-         double cc = cn * c1 - sn * s1;
-         sn = sn * c1 + cn * s1;
-         cn = cc;
-      }
-   }
-}
-
-// Site-major copy of Z for the back-projection operand: per site, the padded
-// l range of Zr then Zi (K2 = 2 NCOL columns), conjugate signs applied and
-// zeros outside |l| <= lmax, hi and lo parts.  A block transposes 64 sites
-// through LDS: coalesced row reads of the l-major tables, contiguous writes.
-constexpr int ZT_SITES = 64;
-constexpr int ZT_MAXK2 = 192;   // 2 * 16 * NB for NB <= 6
-
-__global__ __launch_bounds__(256) void k_mix_zt(int nsp, int lmax, int ncol, const half_t *__restrict__ Zhr,
-                                                const half_t *__restrict__ Zlr, const half_t *__restrict__ Zhi,
-                                                const half_t *__restrict__ Zli, half_t *__restrict__ ZTh,
-                                                half_t *__restrict__ ZTl) {
-   __shared__ __attribute__((aligned(16))) half_t sh[2][ZT_SITES * ZT_MAXK2];
-   const int K2 = 2 * ncol, tid = threadIdx.x, base = blockIdx.x * ZT_SITES;
-   const int ns_blk = min(ZT_SITES, nsp - base);
-   // columns beyond 2 lmax in each half stay zero
-   for (int q = tid; q < ZT_SITES * K2; q += 256) {
-      const int c = q % K2, cc = c < ncol ? c : c - ncol;
-      if (cc > 2 * lmax) sh[0][q] = sh[1][q] = (half_t)0;
-   }
-   const int si = tid & (ZT_SITES - 1);
-   if (si < ns_blk) {
-      const int i = base + si;
-      for (int al = tid >> 6; al <= lmax; al += 4) {
-         const size_t o = (size_t)al * nsp + i;
-         const half_t rh = Zhr[o], rl = Zlr[o], ih = Zhi[o], il = Zli[o];
-         half_t *h = &sh[0][si * K2], *l = &sh[1][si * K2];
-         h[lmax + al] = rh; l[lmax + al] = rl;
-         h[lmax - al] = rh; l[lmax - al] = rl;
-         h[ncol + lmax + al] = ih; l[ncol + lmax + al] = il;
-         if (al > 0) { h[ncol + lmax - al] = -ih; l[ncol + lmax - al] = -il; }
       }
    }
    __syncthreads();
-   const int nchunk = ns_blk * K2 / 8;
+   const int cpr = K2 / 8, nchunk = ZT_SITES * cpr;
    const size_t ob = (size_t)base * K2;
-   for (int q = tid; q < nchunk; q += 256) {
-      *(half8 *)(ZTh + ob + 8 * (size_t)q) = *(const half8 *)&sh[0][8 * q];
-      *(half8 *)(ZTl + ob + 8 * (size_t)q) = *(const half8 *)&sh[1][8 * q];
+   for (int t = si; t < nchunk; t += ZT_SITES) {
+      const int row = t / cpr, c = t - row * cpr;
+      *(half8 *)(ZTh + ob + 8 * (size_t)t) = *(const half8 *)&sh[0][row * RS + 8 * c];
+      *(half8 *)(ZTl + ob + 8 * (size_t)t) = *(const half8 *)&sh[1][row * RS + 8 * c];
    }
 }
 
@@ -832,11 +824,9 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       ProfScope ps(e, "ewald_mfma_prep");
       TabArgs t;
       for (int i = 0; i < 9; i++) t.ks[i] = 2 * PI_D * e->hinv[i];
-      t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = lmax; t.ns = ns; t.nsp = nsp; t.sZ = sZ;
-      k_mix_tables<<<(nsp + 255) / 256, 256, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Zhr,
-                                                             Zlr, Zhi, Zli);
-      k_mix_zt<<<(nsp + ZT_SITES - 1) / ZT_SITES, 256, 0, e->stream>>>(nsp, lmax, NCOL, Zhr, Zlr, Zhi, Zli, ZTh,
-                                                                      ZTl);
+      t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = lmax; t.ns = ns; t.nsp = nsp; t.ncol = NCOL; t.sZ = sZ;
+      k_mix_tables<<<nsp / ZT_SITES, ZT_SITES, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Zhr, Zlr,
+                                                               Zhi, Zli, ZTh, ZTl);
    }
    {
       ProfScope ps(e, "ewald_mfma_sk");
