@@ -36,8 +36,8 @@ PEAK = {"fp64_vector": 78.6e12, "fp32_vector": 157.3e12, "fp16_mfma": 2.5e15}
 # passes on this workload, scaled by the counters' reading of a plain
 # 512 MiB copy (profiles/saltwater132872_eval_forces_hbm_traffic.md).
 MIB = float(1 << 20)
-TRAFFIC_BYTES = {"ewald_mfma_force": (570.26 + 6.08) * MIB, "ewald_mfma_sk": (727.92 + 123.44) * MIB,
-                 "ewald_sfac": (7189.41 + 52.26) * MIB, "ewald_force": (5414.63 + 97.32) * MIB}
+TRAFFIC_BYTES = {"ewald_mfma_force": (657.60 + 3.04) * MIB, "ewald_mfma_sk": (649.87 + 79.72) * MIB,
+                 "ewald_sfac": (7184.41 + 52.26) * MIB, "ewald_force": (5415.98 + 97.32) * MIB}
 
 
 def build_workload(small=False):
