@@ -89,9 +89,15 @@ void mgpu::device_scan_int(mgpu_engine *e, const int *in, int *out, int n) {
 // ------------------------------------------------- column sums (two stages)
 // out[c] = sum over rows of in[r*ncols + c], ncols <= 16, summed in a fixed
 // order (per-block partials, then one block), so results are reproducible.
-static __global__ __launch_bounds__(256) void k_rc1(const double *__restrict__ in, int nrows, int ncols,
-                                                    int chunk, double *__restrict__ tmp) {
+// Column sums of in[nrows][ncols] (ncols <= 16) into out[ncols] in one
+// launch: every block writes its partial, and the last block to finish (an
+// atomic count, reset by that block) adds the partials in block order, so
+// the result does not depend on which block finishes last.
+static __global__ __launch_bounds__(256) void k_rc(const double *__restrict__ in, int nrows, int ncols, int chunk,
+                                                  unsigned *__restrict__ count, double *__restrict__ tmp,
+                                                  double *__restrict__ out) {
    __shared__ double ws[4][16];
+   __shared__ bool last;
    const int r0 = blockIdx.x * chunk, r1 = min(nrows, r0 + chunk);
    double acc[16];
 #pragma unroll
@@ -111,13 +117,18 @@ static __global__ __launch_bounds__(256) void k_rc1(const double *__restrict__ i
    if (threadIdx.x < ncols)
       tmp[(size_t)blockIdx.x * ncols + threadIdx.x] =
           ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
-}
-
-static __global__ void k_rc2(const double *__restrict__ tmp, int nb, int ncols, double *__restrict__ out) {
-   if ((int)threadIdx.x < ncols) {
-      double s = 0.0;
-      for (int b = 0; b < nb; b++) s += tmp[(size_t)b * ncols + threadIdx.x];
-      out[threadIdx.x] = s;
+   __threadfence();
+   __syncthreads();
+   if (threadIdx.x == 0) last = atomicAdd(count, 1u) == gridDim.x - 1;
+   __syncthreads();
+   if (last) {
+      __threadfence();
+      if ((int)threadIdx.x < ncols) {
+         double t = 0.0;
+         for (unsigned b = 0; b < gridDim.x; b++) t += tmp[(size_t)b * ncols + threadIdx.x];
+         out[threadIdx.x] = t;
+      }
+      if (threadIdx.x == 0) *count = 0u;
    }
 }
 
@@ -125,9 +136,12 @@ void mgpu::reduce_cols(mgpu_engine *e, const double *in, int nrows, int ncols, d
    if (ncols > 16) throw StatusError{MGPU_E_ARG, "reduce_cols: more than 16 columns"};
    int nb = std::max(1, std::min(512, (nrows + 2047) / 2048));
    int chunk = (nrows + nb - 1) / nb;
+   // d_rtmp: a completion count in its first slot, then the partials
+   const double *before = e->d_rtmp.p;
    e->d_rtmp.reserve((size_t)nb * ncols + 16);
-   k_rc1<<<nb, 256, 0, e->stream>>>(in, nrows, ncols, chunk, e->d_rtmp.p);
-   k_rc2<<<1, 64, 0, e->stream>>>(e->d_rtmp.p, nb, ncols, out_dev);
+   unsigned *count = (unsigned *)e->d_rtmp.p;
+   if (e->d_rtmp.p != before) zero_async(e, count, 8);   // fresh memory: the count starts at zero
+   k_rc<<<nb, 256, 0, e->stream>>>(in, nrows, ncols, chunk, count, e->d_rtmp.p + 1, out_dev);
 }
 
 // ---------------------------------------------------------------- binning

@@ -16,7 +16,7 @@
 // the work is spread as one thread per (site, part of the shell): many
 // short, independent loops over L2-resident cell-ordered positions keep
 // enough waves in flight to hide the gather latency.  The number of parts
-// (4 to 64) grows as the system shrinks, so small boxes still fill the GPU.
+// (4 to 256) grows as the system shrinks, so small boxes still fill the GPU.
 #include "mgpu_internal.h"
 #include "mgpu_pair.h"
 
@@ -63,9 +63,9 @@ template <> struct Pos<float> {
 
 // One thread per (site i, part q of the full shell): part q takes offsets
 // o = q, q + nq, ... (own cell in part 0) and walks the sites of each
-// neighbour cell.  The nq threads of a site are adjacent lanes of one wave
-// and combine their forces with shuffles; energy and virial go to one
-// partial per wavefront.
+// neighbour cell.  The nq threads of a site are adjacent lanes (and, past
+// 64, adjacent waves of one block) and combine their forces with shuffles
+// and LDS; energy and virial go to one partial per wavefront.
 
 template <class R, int PT, bool COUL>
 __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
@@ -169,11 +169,22 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
       v[1] += (double)w00; v[2] += (double)w01; v[3] += (double)w02;
       v[4] += (double)w11; v[5] += (double)w12; v[6] += (double)w22;
    }
-   // forces: the nq parts of a site sit in adjacent lanes
-   for (int m = 1; m < nq; m <<= 1) {
+   // forces: the nq parts of a site sit in adjacent lanes, and for nq > 64
+   // in adjacent waves of the block, summed in wave order through LDS
+   const int nql = nq < 64 ? nq : 64;
+   for (int m = 1; m < nql; m <<= 1) {
       gx += __shfl_xor(gx, m, 64);
       gy += __shfl_xor(gy, m, 64);
       gz += __shfl_xor(gz, m, 64);
+   }
+   if (nq > 64) {
+      __shared__ double sred[4][3];
+      const int wv = threadIdx.x >> 6;
+      if (lane == 0) { sred[wv][0] = gx; sred[wv][1] = gy; sred[wv][2] = gz; }
+      __syncthreads();
+      if (qtr == 0) {
+         for (int w = wv + 1; w < wv + nq / 64; w++) { gx += sred[w][0]; gy += sred[w][1]; gz += sred[w][2]; }
+      }
    }
    if (have && qtr == 0) {
       const int io = a.corig[p];
@@ -208,10 +219,10 @@ void launch_pt(int pt, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
 
 void mgpu::run_realspace(mgpu_engine *e) {
    int nc = e->ncells;
-   // shell parts per site: at least 4, more until the grid holds about 16
-   // waves per CU
+   // shell parts per site: at least 4, more (up to 256, four waves of one
+   // 256-thread block) until the grid holds about 16 waves per CU
    int lognq = 2;
-   while (lognq < 6 && (long)e->nsites << lognq < 16L * 64 * e->ncu) lognq++;
+   while (lognq < 8 && (long)e->nsites << lognq < 16L * 64 * e->ncu) lognq++;
    const int nthreads = e->nsites << lognq, nblk = (nthreads + 255) / 256, nwaves = nblk * 4;
    e->d_cell_part.reserve((size_t)nwaves * 8 + 8);
    RSArgs a;
