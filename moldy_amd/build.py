@@ -24,6 +24,9 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Wno-unused-variable", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INC, "-I" + CSRC]
+# the FP64 S(k) GEMM keeps its accumulators in VGPRs: in the AGPR form the
+# compiler copies all of them between the two register files every step
+FILE_FLAGS = {"mgpu_ewald.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 SOURCES = ["mgpu_engine.hip", "mgpu_cells.hip", "mgpu_realspace.hip", "mgpu_ewald.hip", "mgpu_dynamics.hip",
            "mgpu_ewald_mixed.hip", "mgpu_glue.hip", "mgpu_adjoint.hip",
            "mgpu_batch.hip"]
@@ -44,7 +47,7 @@ def build(jobs=8, verbose=False):
     for src in SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"

@@ -268,7 +268,7 @@ void mgpu::set_geometry(mgpu_engine *e, const double h[9]) {
          e->hmax = (int)floor(kc / (2 * PI_D) * h[0]);
          e->kmax = (int)floor(kc / (2 * PI_D) * sqrt(h[1] * h[1] + h[4] * h[4]));
          e->lmax = (int)floor(kc / (2 * PI_D) * sqrt(h[2] * h[2] + h[5] * h[5] + h[8] * h[8]));
-         // the S(k) kernel stages exp(i l c*.r), l = 0..lmax, for 32 sites in 64 KB of LDS
+         // the FP64 path is built and tested for l up to 80 (two column groups of the S(k) GEMM)
          if (e->lmax > 80) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large (lmax > 80)"};
          e->rows.clear();
          e->hkl.clear();
@@ -692,7 +692,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cell_nsites); fr(e->d_cell_site_start); fr(e->d_scan_tmp); fr(e->d_corig); fr(e->d_cpos);
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
-   fr(e->d_groups); fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_rtmp_side); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_rtmp_side); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
    fr(e->d_adj); fr(e->d_adj_real);
    fr(e->d_pl_nab); fr(e->d_pl_pbc); fr(e->d_pl_nfnab); fr(e->d_rdf_blk); fr(e->d_rdf_hist); fr(e->d_rdf_bins);
    fr(e->d_mom); fr(e->d_amom); fr(e->d_scal); fr(e->d_trace); fr(e->d_spec_dyn);

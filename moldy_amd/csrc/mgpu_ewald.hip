@@ -7,26 +7,23 @@
 // (ewald.c:557-579).  Framework sites (the last ns - nsitesxf sites) enter
 // S(k) separately so that frame-frame terms are excluded as in the reference.
 //
-// S(k): a workgroup owns a group of consecutive k vectors (at most 256, from
-// at most KROWS (h,k) rows) and a chunk of sites; per tile of sites it stages
-// A = q-free exp(i(h a*.r + k b*.r)) for its rows and the q-weighted
-// exp(i l c*.r) table in LDS, and each thread accumulates one k vector.
-// Chunk partials are reduced in a fixed order (deterministic).
+// S(k): a complex GEMM of the (h,k) rows against the l columns on the FP64
+// matrix cores, per chunk of sites; chunk partials are reduced in a fixed
+// order (deterministic).
 //
 // Forces: one thread per site walks the rows; exp(i l c*.r) is advanced along
-// a row by multiplication with exp(i c*.r), so the inner loop touches no
-// memory except the per-k weights (uniform loads).
+// a row by multiplication with exp(i c*.r), and exp(i k b*.r) along a run of
+// rows of one h, so the inner loop touches no memory except the per-k
+// weights (uniform loads).
 #include "mgpu_internal.h"
 
 #include <algorithm>
+#include <cmath>
 
 using namespace mgpu;
 
 namespace {
 
-constexpr int KROWS = 32;   // rows per S(k) workgroup
-constexpr int TS = 32;      // sites per LDS tile
-constexpr int CHUNK = 8192; // sites per S(k) partial
 constexpr double PI_D = 3.14159265358979323846;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -75,85 +72,119 @@ __global__ void k_trig(TrigArgs a, const double *__restrict__ site, const double
    }
 }
 
-struct SkGroup {
-   int r0, nr, k0, nkg;
-};
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 struct SkArgs {
-   int nk, ns, nsp, nsxf, lmax, nchunks;
+   int nk, ns, nsp, nsxf, nrows, lmax;
+   int nca, ncb;       // site chunks over [0, nsxf) and over the framework sites [nsxf, ns)
+   int nrb;            // row blocks of 64 rows
    const KRow *rows;
-   const SkGroup *groups;
-   const int *krow;
    const double2 *tx, *ty, *tz;
-   double *part;    // [nchunks][nk][4]
+   double *part;       // [nca + ncb][nk][4]: (C, S) of a chunk in slots 0-1 or, framework, 2-3
 };
 
-template <bool FW>
+// S(k) as a complex GEMM on the FP64 matrix cores.  Rows are the (h,k) rows
+// of the k set (A_r(i) = exp(i(h a* + k b*).r_i)), columns the l >= 0 of the
+// q-weighted table Z_l(i) = q_i exp(i l c*.r_i), and the sum runs over a
+// chunk of sites.  Four real products RR = sum Ar Zr, II = sum Ai Zi,
+// RI = sum Ar Zi, IR = sum Ai Zr give both signs of l:
+// S(h,k,+l) = (RR - II) + i(RI + IR) and, as Z_{-l} = conj Z_l (qsincos,
+// ewald.c:108-121), S(h,k,-l) = (RR + II) + i(IR - RI).  A workgroup is four
+// waves, one 16-row tile each, over NC 16-column tiles starting at
+// l = 48 blockIdx.y.  v_mfma_f64_16x16x4f64 takes site i0 + 4s + lane/16 with row (column)
+// lane%16 in its A (B) operand and leaves D[lane/16 + 4v][lane%16] in item
+// v (both checked on gfx950 with integer operands).  The workgroups of one chunk go to one XCD (workgroup ids are dealt
+// round robin over the 8 XCDs), so that the chunk's X, Y and Z rows come from
+// HBM about once.  Each k of the chunk is stored exactly once.
+template <int NC>
 __global__ __launch_bounds__(256) void k_sfac(SkArgs a) {
-   __shared__ double2 sA[KROWS][TS + 1];
-   extern __shared__ double2 sZ[];   // [(lmax+1)][TS+1]
-   const SkGroup g = a.groups[blockIdx.x];
-   const int chunk = blockIdx.y;
-   const int i0 = chunk * CHUNK, i1 = min(a.ns, i0 + CHUNK);
-   const int tid = threadIdx.x;
-   const int kk = g.k0 + tid;
-   const bool active = tid < g.nkg;
-   int rloc = 0, l = 0;
-   if (active) {
-      int r = a.krow[kk];
-      rloc = r - g.r0;
-      KRow R = a.rows[r];
-      l = R.l0 + (kk - R.kstart);
+   const int nchunks = a.nca + a.ncb;
+   const int L = blockIdx.x, s8 = L >> 3;
+   const int chunk = (L & 7) * (nchunks >> 3) + s8 / a.nrb;
+   const int rb = s8 % a.nrb;
+   const bool fwc = chunk >= a.nca;
+   int c0, c1;
+   if (!fwc) {
+      c0 = (int)((long)a.nsxf * chunk / a.nca);
+      c1 = (int)((long)a.nsxf * (chunk + 1) / a.nca);
+   } else {
+      const long nf = a.ns - a.nsxf, c = chunk - a.nca;
+      c0 = a.nsxf + (int)(nf * c / a.ncb);
+      c1 = a.nsxf + (int)(nf * (c + 1) / a.ncb);
    }
-   const int al = l < 0 ? -l : l;
-   const double zsgn = l < 0 ? -1.0 : 1.0;
-   double sc = 0, ss = 0, scf = 0, ssf = 0;
-   for (int t0 = i0; t0 < i1; t0 += TS) {
-      const int nt = min(TS, i1 - t0);
-      __syncthreads();
-      for (int q = tid; q < (a.lmax + 1) * TS; q += 256) {
-         int ll = q / TS, t = q - ll * TS;
-         sZ[ll * (TS + 1) + t] = t < nt ? a.tz[(size_t)ll * a.nsp + t0 + t] : make_double2(0, 0);
-      }
-      for (int q = tid; q < g.nr * TS; q += 256) {
-         int rr = q / TS, t = q - rr * TS;
-         double2 v = make_double2(0, 0);
-         if (t < nt) {
-            KRow R = a.rows[g.r0 + rr];
-            double2 xh = a.tx[(size_t)R.h * a.nsp + t0 + t];
-            int ak = R.k < 0 ? -R.k : R.k;
-            double2 yk = a.ty[(size_t)ak * a.nsp + t0 + t];
-            // qsincos (ewald.c:108-121): k >= 0 uses exp(+i k b*.r), k < 0 the conjugate
-            if (R.k < 0) yk.y = -yk.y;
-            v = make_double2(xh.x * yk.x - xh.y * yk.y, xh.y * yk.x + xh.x * yk.y);
-         }
-         sA[rr][t] = v;
-      }
-      __syncthreads();
-      if (active) {
-         if (!FW || t0 + nt <= a.nsxf) {
-            for (int t = 0; t < nt; t++) {
-               double2 A = sA[rloc][t];
-               double2 Z = sZ[al * (TS + 1) + t];
-               double zi = zsgn * Z.y;
-               sc += A.x * Z.x - A.y * zi;
-               ss += A.y * Z.x + A.x * zi;
-            }
-         } else {
-            for (int t = 0; t < nt; t++) {
-               double2 A = sA[rloc][t];
-               double2 Z = sZ[al * (TS + 1) + t];
-               double zi = zsgn * Z.y;
-               double c = A.x * Z.x - A.y * zi, s = A.y * Z.x + A.x * zi;
-               if (t0 + t < a.nsxf) { sc += c; ss += s; }
-               else { scf += c; ssf += s; }
-            }
-         }
+   const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+   const int rt = rb * 4 + (threadIdx.x >> 6);
+   // this lane's operand row and columns; rows past the end read the last
+   // row and columns past lmax read row lmax: their outputs are never stored
+   const KRow R = a.rows[min(rt * 16 + m, a.nrows - 1)];
+   const double2 *xp = a.tx + (size_t)R.h * a.nsp;
+   const double2 *yp = a.ty + (size_t)(R.k < 0 ? -R.k : R.k) * a.nsp;
+   const double ysg = R.k < 0 ? -1.0 : 1.0;
+   const int lb = 48 * blockIdx.y;
+   const double2 *zp[NC];
+#pragma unroll
+   for (int c = 0; c < NC; c++) zp[c] = a.tz + (size_t)min(lb + m + 16 * c, a.lmax) * a.nsp;
+   dbl4 acc[NC][4];
+#pragma unroll
+   for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc[c][q] = dbl4{0, 0, 0, 0};
+   const int nstep = (c1 - c0 + 3) >> 2;
+   const int ilast = max(c1 - 1, 0);
+   int i = c0 + kq;
+   // operands one step ahead; indexes clamped into the chunk, the sites past
+   // its end weighted by zero
+   int ic = min(i, ilast);
+   double2 xn = xp[ic], yn = yp[ic], zn[NC];
+#pragma unroll
+   for (int c = 0; c < NC; c++) zn[c] = zp[c][ic];
+   for (int st = 0; st < nstep; st++) {
+      const double2 x = xn, y = yn;
+      double2 z[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) z[c] = zn[c];
+      const double wgt = i < c1 ? 1.0 : 0.0;
+      i += 4;
+      ic = min(i, ilast);
+      xn = xp[ic];
+      yn = yp[ic];
+#pragma unroll
+      for (int c = 0; c < NC; c++) zn[c] = zp[c][ic];
+      const double yi = ysg * y.y;
+      const double ar = (x.x * y.x - x.y * yi) * wgt, ai = (x.y * y.x + x.x * yi) * wgt;
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+         acc[c][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, z[c].x, acc[c][0], 0, 0, 0);
+         acc[c][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, z[c].y, acc[c][1], 0, 0, 0);
+         acc[c][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, z[c].y, acc[c][2], 0, 0, 0);
+         acc[c][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, z[c].x, acc[c][3], 0, 0, 0);
       }
    }
-   if (active) {
-      double *p = a.part + ((size_t)chunk * a.nk + kk) * 4;
-      p[0] = sc; p[1] = ss; p[2] = scf; p[3] = ssf;
+   // store: item v of this lane is row rt*16 + kq + 4v, column l = lb + m + 16c
+#pragma unroll
+   for (int v = 0; v < 4; v++) {
+      const int rr = rt * 16 + kq + 4 * v;
+      if (rr >= a.nrows) continue;
+      const KRow Q = a.rows[rr];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+         const int l = lb + m + 16 * c;
+         if (l > a.lmax) continue;
+         const double RR = acc[c][0][v], II = acc[c][1][v], RI = acc[c][2][v], IR = acc[c][3][v];
+         double2 *p;
+         if (l >= Q.l0 && l < Q.l0 + Q.nl) {
+            p = (double2 *)(a.part + ((size_t)chunk * a.nk + Q.kstart + (l - Q.l0)) * 4);
+            const double2 val = make_double2(RR - II, RI + IR), zero = make_double2(0, 0);
+            p[0] = fwc ? zero : val;
+            p[1] = fwc ? val : zero;
+         }
+         if (l > 0 && -l >= Q.l0 && -l < Q.l0 + Q.nl) {
+            p = (double2 *)(a.part + ((size_t)chunk * a.nk + Q.kstart + (-l - Q.l0)) * 4);
+            const double2 val = make_double2(RR + II, IR - RI), zero = make_double2(0, 0);
+            p[0] = fwc ? zero : val;
+            p[1] = fwc ? val : zero;
+         }
+      }
    }
 }
 
@@ -219,6 +250,7 @@ struct EfArgs {
    const int *rpart;        // [npart+1] row ranges of the partitions
    const double *w;
    const double2 *tx, *ty, *tz, *tz1;
+   const double2 *ty1;      // row k = 1 of ty (row 0 when kmax = 0, where it is never used)
    double *fpart;           // [npart][3][nsp]
 };
 
@@ -233,34 +265,41 @@ __global__ __launch_bounds__(256) void k_ewald_force(EfArgs a) {
    const int wo = fw ? 2 : 0;
    const double2 z1 = a.tz1[i];
    double fx = 0, fy = 0, fz = 0;
-   int hcur = -1;
-   double2 xh = make_double2(1, 0);
+   int hcur = -1, kcur = 0;
+   double2 xh = make_double2(1, 0), A = make_double2(1, 0);
+   const double2 y1 = a.ty1[i];   // exp(i b*.r)
    for (int r = a.rpart[part]; r < a.rpart[part + 1]; r++) {
       const KRow R = a.rows[r];
-      if (R.h != hcur) {
-         hcur = R.h;
-         xh = a.tx[(size_t)R.h * a.nsp + i];
+      if (R.h == hcur && R.k == kcur + 1) {
+         A = cmul(A, y1);                 // next k of the same h: exp(i (h a* + k b*).r) by recurrence
+      } else {
+         if (R.h != hcur) xh = a.tx[(size_t)R.h * a.nsp + i];
+         int ak = R.k < 0 ? -R.k : R.k;
+         double2 yk = a.ty[(size_t)ak * a.nsp + i];
+         if (R.k < 0) yk.y = -yk.y;
+         A = cmul(xh, yk);
       }
-      int ak = R.k < 0 ? -R.k : R.k;
-      double2 yk = a.ty[(size_t)ak * a.nsp + i];
-      if (R.k < 0) yk.y = -yk.y;
-      double2 A = cmul(xh, yk);
+      hcur = R.h;
+      kcur = R.k;
       int al0 = R.l0 < 0 ? -R.l0 : R.l0;
       double2 z = a.tz[(size_t)al0 * a.nsp + i];
       if (R.l0 < 0) z.y = -z.y;
       const double *w = a.w + 4 * (size_t)R.kstart + wo;
-      double kzl = 0;
+      // along the run of l: e = q exp(i k.r) advanced by exp(i c*.r), and
+      // sum_l kz_l fc_l = (kzt + l0 c2) sum fc + c2 sum (l - l0) fc
+      double2 e = cmul(A, z);
+      double sf = 0, sq = 0, dq = 0;
       for (int q = 0; q < R.nl; q++) {
-         double2 e = cmul(A, z);             // q cos(k.r), q sin(k.r)
-         double wc = w[4 * q], wsn = w[4 * q + 1];
-         double fc = e.y * wc - e.x * wsn;
-         int l = R.l0 + q;
-         kzl = R.kzt + l * a.c2;
-         fx += R.kx * fc;
-         fy += R.ky * fc;
-         fz += kzl * fc;
-         z = cmul(z, z1);
+         const double wc = w[4 * q], wsn = w[4 * q + 1];
+         const double fc = e.y * wc - e.x * wsn;
+         sf += fc;
+         sq += dq * fc;
+         dq += 1.0;
+         e = cmul(e, z1);
       }
+      fx += R.kx * sf;
+      fy += R.ky * sf;
+      fz += (R.kzt + R.l0 * a.c2) * sf + a.c2 * sq;
    }
    double *fp = a.fpart + (size_t)part * 3 * a.nsp;
    fp[i] = fx;
@@ -299,26 +338,10 @@ static void ewald_tables(mgpu_engine *e, const double *w) {
        t, e->d_site.p, w, e->d_tx.p, e->d_ty.p, e->d_tz.p, e->d_tz.p + (size_t)(e->lmax + 1) * nsp);
 }
 
-// k groups: <= 256 k vectors, <= KROWS rows each, and row partitions of
-// the force pass with about equal numbers of k vectors; both are host
-// tables uploaded when the k set changes
+// Row partitions of the force pass with about equal numbers of k vectors, a
+// host table uploaded when the k set changes
 static void ewald_plan(mgpu_engine *e) {
    if (e->ew_plan_version != e->geom_version) {
-      std::vector<SkGroup> groups;
-      int r = 0;
-      while (r < e->nrows) {
-         SkGroup g{r, 0, e->rows[r].kstart, 0};
-         while (r < e->nrows && g.nr < KROWS && g.nkg + e->rows[r].nl <= 256) {
-            g.nkg += e->rows[r].nl;
-            g.nr++;
-            r++;
-         }
-         groups.push_back(g);
-      }
-      static_assert(sizeof(SkGroup) == sizeof(int4), "SkGroup layout");
-      e->d_groups.reserve(groups.size() + 1);
-      HIPCHK(hipMemcpyAsync(e->d_groups.p, groups.data(), sizeof(SkGroup) * groups.size(),
-                            hipMemcpyHostToDevice, e->stream));
       const int npart = std::max(1, std::min(e->tune_npart, e->nrows / 8));
       std::vector<int> rp(npart + 1, e->nrows);
       rp[0] = 0;
@@ -329,31 +352,63 @@ static void ewald_plan(mgpu_engine *e) {
       e->d_rpart.reserve(npart + 1);
       HIPCHK(hipMemcpyAsync(e->d_rpart.p, rp.data(), sizeof(int) * (npart + 1), hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
-      e->ew_ngroups = (int)groups.size();
       e->ew_npart = npart;
       e->ew_plan_version = e->geom_version;
    }
+}
+
+// S(k) chunk partials of the current trig tables into d_sk, sites from nsxf
+// on counted as framework; returns the number of chunks.  About four
+// workgroups per CU in all, chunks of at least 256 sites, a multiple of 8
+// chunks (one XCD each).
+static int sfac_partials(mgpu_engine *e, int nsxf) {
+   const int ns = e->nsites, nrb = (e->nrows + 63) / 64;
+   int nch = (4 * e->ncu + nrb - 1) / std::max(nrb, 1);
+   nch = std::min(nch, std::max(8, ns / 256));
+   nch = std::max(8, (nch + 7) / 8 * 8);
+   int ncb = 0;
+   if (nsxf == 0)
+      ncb = nch;
+   else if (nsxf < ns)
+      ncb = std::min(nch - 1, std::max(1, (int)std::lround((double)nch * (ns - nsxf) / ns)));
+   e->d_sk.reserve((size_t)nch * e->nk * 4 + 4);
+   SkArgs s;
+   s.nk = e->nk; s.ns = ns; s.nsp = e->nsp; s.nsxf = nsxf; s.nrows = e->nrows; s.lmax = e->lmax;
+   s.nca = nch - ncb; s.ncb = ncb; s.nrb = nrb;
+   s.rows = e->d_rows.p; s.tx = e->d_tx.p; s.ty = e->d_ty.p; s.tz = e->d_tz.p; s.part = e->d_sk.p;
+   const int ncol = e->lmax + 1;
+   const dim3 g(nrb * nch, (ncol + 47) / 48);
+   switch (std::min(3, (ncol + 15) / 16)) {
+   case 1: k_sfac<1><<<g, 256, 0, e->stream>>>(s); break;
+   case 2: k_sfac<2><<<g, 256, 0, e->stream>>>(s); break;
+   default: k_sfac<3><<<g, 256, 0, e->stream>>>(s); break;
+   }
+   return nch;
+}
+
+// Arguments of the back-projection over the current trig tables, weights
+// d_wk and partials d_fpart; sites from nsxf on use the framework weights
+static EfArgs force_args(mgpu_engine *e, int nsxf) {
+   const size_t nsp = e->nsp;
+   EfArgs ef;
+   ef.ns = e->nsites; ef.nsp = e->nsp; ef.nsxf = nsxf; ef.nrows = e->nrows;
+   ef.c2 = 2 * PI_D * e->hinv[8];
+   ef.rows = e->d_rows.p; ef.rpart = e->d_rpart.p; ef.w = e->d_wk.p;
+   ef.tx = e->d_tx.p; ef.ty = e->d_ty.p; ef.tz = e->d_tz.p;
+   ef.tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * nsp;
+   ef.ty1 = e->d_ty.p + (e->kmax > 0 ? nsp : 0);
+   ef.fpart = e->d_fpart.p;
+   return ef;
 }
 
 void mgpu::run_ewald_fp64(mgpu_engine *e) {
    ewald_plan(e);
    ewald_tables(e, e->d_chg_site.p);
    const size_t nsp = e->nsp;
-   const double2 *tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * nsp;
-   int nchunks = (e->nsites + CHUNK - 1) / CHUNK;
-   e->d_sk.reserve((size_t)nchunks * e->nk * 4 + 4);
-   SkArgs s;
-   s.nk = e->nk; s.ns = e->nsites; s.nsp = e->nsp; s.nsxf = e->nsitesxf; s.lmax = e->lmax;
-   s.nchunks = nchunks; s.rows = e->d_rows.p; s.groups = (const SkGroup *)e->d_groups.p; s.krow = e->d_krow.p;
-   s.tx = e->d_tx.p; s.ty = e->d_ty.p; s.tz = e->d_tz.p; s.part = e->d_sk.p;
-   size_t shz = sizeof(double2) * (e->lmax + 1) * (TS + 1);
+   int nchunks;
    {
       ProfScope ps(e, "ewald_sfac");
-      dim3 g(e->ew_ngroups, nchunks);
-      if (e->nsitesxf < e->nsites)
-         k_sfac<true><<<g, 256, shz, e->stream>>>(s);
-      else
-         k_sfac<false><<<g, 256, shz, e->stream>>>(s);
+      nchunks = sfac_partials(e, e->nsitesxf);
    }
    int nkb = (e->nk + 255) / 256;
    e->d_wk.reserve((size_t)e->nk * 4 + 4);
@@ -373,11 +428,7 @@ void mgpu::run_ewald_fp64(mgpu_engine *e) {
    reduce_cols(e, e->d_red.p, nkb, 8, e->d_out.p + OUT_RECIP);
    const int npart = e->ew_npart;
    e->d_fpart.reserve((size_t)npart * 3 * nsp);
-   EfArgs ef;
-   ef.ns = e->nsites; ef.nsp = e->nsp; ef.nsxf = e->nsitesxf; ef.nrows = e->nrows;
-   ef.c2 = ek.c2; ef.rows = e->d_rows.p; ef.rpart = e->d_rpart.p; ef.w = e->d_wk.p;
-   ef.tx = e->d_tx.p; ef.ty = e->d_ty.p; ef.tz = e->d_tz.p; ef.tz1 = tz1;
-   ef.fpart = e->d_fpart.p;
+   const EfArgs ef = force_args(e, e->nsitesxf);
    {
       ProfScope ps(e, "ewald_force");
       dim3 g((e->nsites + 255) / 256, npart);
@@ -545,15 +596,7 @@ __global__ void k_jvp_phi(int ns, int nsp, const int *__restrict__ stype, const 
 // sum_i w_i exp(i k.r_i) for every k into cs[nk]
 static void weighted_sf(mgpu_engine *e, const double *w, double2 *cs) {
    ewald_tables(e, w);
-   int nchunks = (e->nsites + CHUNK - 1) / CHUNK;
-   e->d_sk.reserve((size_t)nchunks * e->nk * 4 + 4);
-   SkArgs s;
-   s.nk = e->nk; s.ns = e->nsites; s.nsp = e->nsp; s.nsxf = e->nsites; s.lmax = e->lmax;
-   s.nchunks = nchunks; s.rows = e->d_rows.p; s.groups = (const SkGroup *)e->d_groups.p; s.krow = e->d_krow.p;
-   s.tx = e->d_tx.p; s.ty = e->d_ty.p; s.tz = e->d_tz.p; s.part = e->d_sk.p;
-   size_t shz = sizeof(double2) * (e->lmax + 1) * (TS + 1);
-   dim3 g(e->ew_ngroups, nchunks);
-   k_sfac<false><<<g, 256, shz, e->stream>>>(s);
+   const int nchunks = sfac_partials(e, e->nsites);
    k_sf_sum<<<(e->nk + 255) / 256, 256, 0, e->stream>>>(e->nk, nchunks, e->d_sk.p, cs);
 }
 
@@ -564,11 +607,7 @@ static void backproject(mgpu_engine *e, const double *w, const double4 *kc, cons
    ewald_tables(e, w);
    const int npart = e->ew_npart;
    e->d_fpart.reserve((size_t)npart * 3 * e->nsp);
-   EfArgs ef;
-   ef.ns = e->nsites; ef.nsp = e->nsp; ef.nsxf = e->nsites; ef.nrows = e->nrows;
-   ef.c2 = 2 * PI_D * e->hinv[8]; ef.rows = e->d_rows.p; ef.rpart = e->d_rpart.p; ef.w = e->d_wk.p;
-   ef.tx = e->d_tx.p; ef.ty = e->d_ty.p; ef.tz = e->d_tz.p; ef.tz1 = e->d_tz.p + (size_t)(e->lmax + 1) * e->nsp;
-   ef.fpart = e->d_fpart.p;
+   const EfArgs ef = force_args(e, e->nsites);
    dim3 g((e->nsites + 255) / 256, npart);
    k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
    k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p, out);

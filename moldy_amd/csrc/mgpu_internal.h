@@ -154,7 +154,6 @@ struct mgpu_engine {
    mgpu::DBuf<int> d_krow;                      // row of each k
    mgpu::DBuf<double2> d_tx, d_ty, d_tz;        // trig tables [n+1][nsp]
    mgpu::DBuf<double> d_sk;                     // S(k) partial sums
-   mgpu::DBuf<int4> d_groups;                   // S(k) workgroup -> k rows
    mgpu::DBuf<double> d_wk;                     // per-k weights
    mgpu::DBuf<int> d_rpart;                     // row partitions of the force pass
    mgpu::DBuf<double> d_fpart;                  // per-partition site forces
@@ -170,7 +169,7 @@ struct mgpu_engine {
    // geometry version changes, so a steady-state evaluation is launches only
    int geom_version = 0;
    int ew_plan_version = -1, mix_plan_version = -1;
-   int ew_ngroups = 0, ew_npart = 0;
+   int ew_npart = 0;
    // launch shapes chosen by the autotuner (mgpu_autotune)
    int ncu = 256;                               // compute units of the device
    int tune_sk_blocks = 1024;                   // most S(k) MFMA workgroups (2 per CU resident)
