@@ -11,9 +11,13 @@
 //     P[i,r] = sum_l Z[i,l] conj W[r,l],   Q[i,r] = sum_l l Z[i,l] conj W[r,l],
 // (plus kzt(r) P in the z component).  Both products run as
 // v_mfma_f32_16x16x32_f16 with every operand split into FP16 hi + lo parts
-// (a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi) and FP32 accumulation.  The l
-// range is padded to 16 NB columns; entries outside the reference's sphere
-// carry zero weight, so the result is the reference's sum over its k set.
+// (a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi) and FP32 accumulation.  The
+// columns are l >= 0 only, padded to 16 NB: as Z[i,-l] = conj Z[i,l]
+// (qsincos, ewald.c:108-121), the four real products of the forward GEMM
+// give S[r,+l] and S[r,-l] together, and the back-projection folds W[r,+l]
+// and W[r,-l] into the weights of Zr[i,l] and Zi[i,l].  Entries outside the
+// reference's sphere carry zero weight, so the result is the reference's
+// sum over its k set.
 // The reduction over site slices, energy, stress and weights are FP64.
 //
 // Framework sites: site slices never mix framework and non-framework sites,
@@ -57,10 +61,10 @@ struct TabArgs {
 // exp(i n k*.r) by the reference's recurrence (ewald.c:148-193), FP64, then
 // FP32 for the x and y axes and split FP16 for q exp(i l c*.r): l-major
 // tables X[n][site], Y[n][site], Z[n][site] (the forward operands) and a
-// site-major copy of Z for the back-projection operand -- per site the
-// padded l range of Zr then Zi (K2 = 2 NCOL columns), conjugate signs
-// applied, zeros outside |l| <= lmax, hi and lo parts -- assembled in LDS
-// and written as contiguous rows.  One wave of 64 sites per block.
+// site-major copy of Z for the back-projection operand -- per site Zr then
+// Zi for l = 0 .. NCOL-1 (K2 = 2 NCOL columns), zeros past lmax, hi and lo
+// parts -- assembled in LDS and written as contiguous rows.  One wave of 64
+// sites per block.
 constexpr int ZT_SITES = 64;
 constexpr int ZT_MAXK2 = 192;   // 2 * 16 * NB for NB <= 6
 
@@ -77,7 +81,7 @@ __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__re
    half_t *rh = &sh[0][si * RS], *rl = &sh[1][si * RS];
    for (int c = 0; c < K2; c++) {
       const int cc = c < ncol ? c : c - ncol;
-      if (i >= a.ns || cc > 2 * a.lmax) rh[c] = rl[c] = (half_t)0;
+      if (i >= a.ns || cc > a.lmax) rh[c] = rl[c] = (half_t)0;
    }
    if (i >= a.ns) {
       for (int n = 0; n <= a.hmax; n++) X[(size_t)n * a.nsp + i] = make_float2(0, 0);
@@ -87,7 +91,6 @@ __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__re
          Zhr[o] = Zlr[o] = Zhi[o] = Zli[o] = (half_t)0;
       }
    } else {
-      const int lm = a.lmax;
       double x = site[i], y = site[a.nsp + i], z = site[2 * a.nsp + i], q = chg[i] * a.sZ;
       for (int ax = 0; ax < 3; ax++) {
          int nmax = ax == 0 ? a.hmax : (ax == 1 ? a.kmax : a.lmax);
@@ -104,10 +107,8 @@ __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__re
                split2(q * cn, hr, lr);
                split2(q * sn, hi, li);
                Zhr[o] = hr; Zlr[o] = lr; Zhi[o] = hi; Zli[o] = li;
-               rh[lm + n] = hr; rl[lm + n] = lr;
-               rh[lm - n] = hr; rl[lm - n] = lr;
-               rh[ncol + lm + n] = hi; rl[ncol + lm + n] = li;
-               if (n > 0) { rh[ncol + lm - n] = -hi; rl[ncol + lm - n] = -li; }
+               rh[n] = hr; rl[n] = lr;
+               rh[ncol + n] = hi; rl[ncol + n] = li;
             }
             double cc = cn * c1 - sn * s1;
             sn = sn * c1 + cn * s1;
@@ -132,7 +133,7 @@ struct SkArgs {
    const float2 *X, *Y;
    const half_t *Zhr, *Zlr, *Zhi, *Zli;
    const int4 *slices;    // (i0 aligned to 32, i1, valid lo, valid hi)
-   float *part;           // [nslices][ntile][Re|Im][FROWS][16 NB]
+   float *part;           // [nslices][ntile][Re S+l | Im S+l | Re S-l | Im S-l][FROWS][16 NB]
 };
 
 // Eight waves over a tile of FROWS = 128 rows: wave w takes the A part
@@ -203,19 +204,16 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
       *(half8 *)&sA[buf][2][rr][8 * sg] = aih;
       *(half8 *)&sA[buf][3][rr][8 * sg] = ail;
    };
-   // Z tile staging: chunk q = (part, hi/lo, column, 8 sites c).  The tables
-   // hold |l|; exp(-i l c*.r) = conj, so the Zi columns with l < 0 change
-   // sign at the store.  Padding columns (|l| > lmax) read row lmax: the
-   // reduction never reads them.
+   // Z tile staging: chunk q = (part, hi/lo, column l, 8 sites c).  Padding
+   // columns (l > lmax) read row lmax: the reduction never reads them.
    half8 bst[BPT];
    auto load_b = [&](int i0) {
 #pragma unroll
       for (int j = 0; j < BPT; j++) {
          const int q = tid + 512 * j < BCH ? tid + 512 * j : BCH - 1;
          const int c = q & 3, col = (q >> 2) % NCOL, hl = (q / (4 * NCOL)) & 1, part = q / (8 * NCOL);
-         const int l = col - a.lmax, al = l < 0 ? -l : l;
          const half_t *tab = part ? (hl ? a.Zli : a.Zhi) : (hl ? a.Zlr : a.Zhr);
-         bst[j] = *(const half8 *)(tab + (size_t)(al < a.lmax ? al : a.lmax) * a.nsp + i0 + 8 * c);
+         bst[j] = *(const half8 *)(tab + (size_t)(col < a.lmax ? col : a.lmax) * a.nsp + i0 + 8 * c);
       }
    };
    auto store_b = [&](int buf) {
@@ -289,42 +287,43 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
       store_b(buf ^ 1);
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    }
-   // Re S = Ar Zr - Ai Zi and Im S = Ar Zi + Ai Zr: the Ai waves hand their
-   // products to the Ar waves through LDS (the A tile is free after the
-   // last barrier), one row half at a time, and the Ar waves store the two
-   // parts.  Zi columns with l < 0 take the conjugate sign here.
+   // With RR = Ar Zr, II = Ai Zi, RI = Ar Zi, IR = Ai Zr:
+   //   S[+l] = (RR - II) + i (RI + IR),   S[-l] = (RR + II) + i (IR - RI).
+   // The Ai waves hand their products (IR, II) to the Ar waves through LDS
+   // (the A tile is free after the last barrier), one row half at a time;
+   // the (Ar, Zr) wave stores both real parts, the (Ar, Zi) wave both
+   // imaginary parts.
    float *xch = (float *)&sA[0][0][0][0];   // [wn][64 rows][NCOL]
    static_assert(sizeof(sA) >= sizeof(float) * 2 * 64 * NCOL, "exchange tile");
 #pragma unroll
    for (int half = 0; half < 2; half++) {
       if (rh == half && wm == 1) {
 #pragma unroll
-         for (int n = 0; n < NB; n++) {
-            const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
+         for (int n = 0; n < NB; n++)
 #pragma unroll
             for (int m = 0; m < 4; m++)
 #pragma unroll
                for (int j = 0; j < 4; j++)
-                  xch[(wn * 64 + m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = sgn * acc[m][n][j];
-         }
+                  xch[(wn * 64 + m * 16 + 4 * g + j) * NCOL + n * 16 + lc] = acc[m][n][j];
       }
       __syncthreads();
       if (rh == half && wm == 0) {
-         // (Ar, Zr) pairs with (Ai, Zi) for Re; (Ar, Zi) with (Ai, Zr) for Im
-         float *out = a.part + ((size_t)(slice * a.ntile + tile) * 2 + wn) * FROWS * NCOL + (size_t)64 * rh * NCOL;
+         // (Ar, Zr) pairs with (Ai, Zi) for the real parts, (Ar, Zi) with
+         // (Ai, Zr) for the imaginary parts
+         float *outp = a.part + ((size_t)(slice * a.ntile + tile) * 4 + wn) * FROWS * NCOL + (size_t)64 * rh * NCOL;
+         float *outm = outp + (size_t)2 * FROWS * NCOL;
          const float *oth = xch + (size_t)(wn ^ 1) * 64 * NCOL;
 #pragma unroll
-         for (int n = 0; n < NB; n++) {
-            const float sgn = (wn && n * 16 + lc < a.lmax) ? -1.f : 1.f;
+         for (int n = 0; n < NB; n++)
 #pragma unroll
             for (int m = 0; m < 4; m++)
 #pragma unroll
                for (int j = 0; j < 4; j++) {
                   const int o = (m * 16 + 4 * g + j) * NCOL + n * 16 + lc;
-                  const float mine = sgn * acc[m][n][j];
-                  out[o] = wn ? mine + oth[o] : mine - oth[o];
+                  const float mine = acc[m][n][j], other = oth[o];
+                  outp[o] = wn ? mine + other : mine - other;
+                  outm[o] = wn ? other - mine : mine + other;
                }
-         }
       }
       __syncthreads();
    }
@@ -337,7 +336,7 @@ struct RedArgs {
    const KRow *rows;
    const int *slice_fw;
    const float *part;
-   double *Wd, *Wnd;      // dense [nrows][ncol][2]
+   double *Wd, *Wnd;      // dense [nrows][+l | -l][ncol][2]
    double *w;             // per k: Wc, Ws, Wcn, Wsn
    double *bpart;         // per block: 7 sums
    unsigned *maxbits;     // [0] max|W|, [1] max|l W| as float bits
@@ -348,11 +347,13 @@ __global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
    const int idx = blockIdx.x * 256 + threadIdx.x;
    double v[7] = {0, 0, 0, 0, 0, 0, 0};
    float mw = 0.f, mq = 0.f;
-   if (idx < a.nrows * a.ncol) {
-      const int r = idx / a.ncol, c = idx - r * a.ncol, l = c - a.lmax;
+   // one thread per row and signed column: j < ncol is l = j, else l = ncol - j
+   if (idx < a.nrows * 2 * a.ncol) {
+      const int r = idx / (2 * a.ncol), j = idx - r * 2 * a.ncol, neg = j >= a.ncol;
+      const int c = j - neg * a.ncol, l = neg ? -c : c;
       const KRow R = a.rows[r];
       double wr = 0, wi = 0, wnr = 0, wni = 0;
-      if (l >= R.l0 && l < R.l0 + R.nl) {
+      if ((!neg || c > 0) && l >= R.l0 && l < R.l0 + R.nl) {
          const int tile = r / FROWS, rr = r - tile * FROWS;
          double scn = 0, ssn = 0, scf = 0, ssf = 0;
          const size_t pl = (size_t)FROWS * a.ncol;
@@ -361,8 +362,8 @@ __global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
             if (fw) { scf += (double)pr; ssf += (double)pi; }
             else { scn += (double)pr; ssn += (double)pi; }
          };
-         const float *pb = a.part + (size_t)tile * 2 * pl + rr * a.ncol + c;
-         const size_t sstride = (size_t)a.ntile * 2 * pl;
+         const float *pb = a.part + ((size_t)tile * 4 + 2 * neg) * pl + rr * a.ncol + c;
+         const size_t sstride = (size_t)a.ntile * 4 * pl;
          int s = 0;
          for (; s + 8 <= a.nslices; s += 8) {
             float v[8][2];
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
          double *wk = a.w + 4 * (size_t)k;
          wk[0] = wr; wk[1] = wi; wk[2] = wnr; wk[3] = wni;
          mw = (float)fmax(fmax(fabs(wr), fabs(wi)), fmax(fabs(wnr), fabs(wni)));
-         mq = mw * (float)(l < 0 ? -l : l);
+         mq = mw * (float)c;
       }
       double *d = a.Wd + 2 * (size_t)idx;
       d[0] = wr; d[1] = wi;
@@ -436,27 +437,34 @@ __device__ __forceinline__ float pow2_scale(unsigned bits, float target) {
 // ------------------- weight operand tables for the back-projection
 struct WtArgs {
    int nrows, nrows16, ncol, lmax;
-   const double *Wd;
+   const double *Wd;      // [nrows][+l | -l][ncol][2]
    half_t *Wt;            // [4 kinds][2 hi/lo][nrows16][2 ncol]
    const unsigned *maxbits;
 };
 
+// P = sum_l Z[l] conj W[l] over l = -lmax..lmax, with Z[-l] = conj Z[l]:
+// for l >= 0, W+ = W[r,+l] and W- = W[r,-l] (zero at l = 0),
+//   Pr = sum Zr (W+r + W-r) + Zi (W+i - W-i),  Pi = sum -Zr (W+i + W-i) + Zi (W+r - W-r),
+// and Q = sum_l l Z[l] conj W[l] likewise,
+//   Qr = sum l (Zr (W+r - W-r) + Zi (W+i + W-i)),  Qi = sum l (Zr (W-i - W+i) + Zi (W+r + W-r)).
 __global__ void k_mix_wtab(WtArgs a) {
    const int K2 = 2 * a.ncol;
    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
    if (idx >= a.nrows16 * K2) return;
    const int r = idx / K2, kk = idx - r * K2;
-   const int part = kk >= a.ncol, c = kk - part * a.ncol, l = c - a.lmax;
-   double wr = 0, wi = 0;
+   const int part = kk >= a.ncol, l = kk - part * a.ncol;
+   double wpr = 0, wpi = 0, wmr = 0, wmi = 0;
    if (r < a.nrows) {
-      wr = a.Wd[2 * ((size_t)r * a.ncol + c)];
-      wi = a.Wd[2 * ((size_t)r * a.ncol + c) + 1];
+      const double *d = a.Wd + 2 * (size_t)r * K2;
+      wpr = d[2 * l];
+      wpi = d[2 * l + 1];
+      wmr = d[2 * (a.ncol + l)];
+      wmi = d[2 * (a.ncol + l) + 1];
    }
    const double sW = pow2_scale(a.maxbits[0], 8192.f), sQ = pow2_scale(a.maxbits[1], 8192.f);
-   // P = sum_l Z conj(W): Pr = Zr Wr + Zi Wi, Pi = Zi Wr - Zr Wi; Q likewise with l W
    double v[4];
-   if (!part) { v[0] = wr; v[1] = -wi; v[2] = l * wr; v[3] = -l * wi; }
-   else { v[0] = wi; v[1] = wr; v[2] = l * wi; v[3] = l * wr; }
+   if (!part) { v[0] = wpr + wmr; v[1] = -(wpi + wmi); v[2] = l * (wpr - wmr); v[3] = l * (wmi - wpi); }
+   else { v[0] = wpi - wmi; v[1] = wpr - wmr; v[2] = l * (wpi + wmi); v[3] = l * (wpr + wmr); }
    for (int kind = 0; kind < 4; kind++) {
       const double s = kind < 2 ? sW : sQ;
       half_t hi, lo;
@@ -738,8 +746,8 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const int ns = e->nsites, nsp = e->nsp, lmax = e->lmax;
-   const int NB = (2 * lmax + 1 + 15) / 16;
-   if (NB > 6) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (lmax > 47)"};
+   const int NB = (lmax + 1 + 15) / 16;
+   if (NB > 6) throw StatusError{MGPU_E_CAPACITY, "k-space cutoff too large for the MFMA path (lmax > 95)"};
    const int NCOL = 16 * NB, K2 = 2 * NCOL;
    const int nrows = e->nrows;
    const int ntile = (nrows + FROWS - 1) / FROWS;
@@ -769,9 +777,9 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const size_t oY = take(sizeof(float2) * (e->kmax + 1) * (size_t)nsp);
    const size_t oZ = take(sizeof(half_t) * 4 * (lmax + 1) * (size_t)nsp);
    const size_t oZT = take(sizeof(half_t) * 2 * (size_t)K2 * nsp);
-   const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 2 * FROWS * NCOL);
-   const size_t oWd = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
-   const size_t oWn = take(sizeof(double) * 2 * (size_t)nrows * NCOL);
+   const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 4 * FROWS * NCOL);
+   const size_t oWd = take(sizeof(double) * 2 * (size_t)nrows * K2);
+   const size_t oWn = take(sizeof(double) * 2 * (size_t)nrows * K2);
    const size_t oWt = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
    const size_t oWtn = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
    const size_t oHK = take(sizeof(int2) * nrows64);
@@ -779,7 +787,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const size_t oSl = take(sizeof(int4) * nslices);
    const size_t oSf = take(sizeof(int) * nslices);
    const size_t oMx = take(sizeof(unsigned) * 4);
-   const int nred = (nrows * NCOL + 255) / 256;
+   const int nred = (nrows * K2 + 255) / 256;
    const size_t oBp = take(sizeof(double) * 8 * (size_t)nred);
    const int nparts = std::max(1, std::min(e->tune_force_parts, nrows16 / 16));
    const size_t oFp = take(sizeof(float) * 3 * (size_t)nparts * nsp);
