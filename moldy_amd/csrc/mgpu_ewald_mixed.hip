@@ -490,20 +490,21 @@ struct FArgs {
    float *fpart;            // [nparts][3][nsp] otherwise
 };
 
-typedef __attribute__((address_space(3))) void lds_void;
-typedef const __attribute__((address_space(1))) void global_void;
 
 // Four waves of 32 sites each (two m tiles of 16), 128 sites a workgroup and
-// two workgroups per CU, so one workgroup's epilogue runs beside the other's
-// MFMAs.  The weight tile (16 rows, 4 kinds x hi/lo, K2 columns) is single-
-// buffered in LDS and filled by LDS DMA (global_load_lds_dwordx4) during the
-// epilogue: no registers stage it.
+// two workgroups per CU (NB <= 3), so one workgroup's epilogue runs beside
+// the other's MFMAs.  The weight tiles (16 rows, 4 kinds x hi/lo, K2
+// columns) are double-buffered in LDS and staged through registers: tile
+// t + 2 is loaded while tile t is multiplied and written to LDS after it,
+// so its loads have a whole tile to land and one barrier per tile orders
+// the buffers.
 template <int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_force(FArgs a) {
    constexpr int NCOL = 16 * NB, K2 = 2 * NCOL;
-   constexpr int CR = K2 / 8 + 1;      // 16-byte chunks per LDS row, the last one padding
-   constexpr int NINST = 2 * CR;       // 1 KB wave-wide DMA loads per tile (128 CR chunks)
-   __shared__ __attribute__((aligned(16))) half_t sW[4][2][16][K2 + 8];
+   constexpr int RW = K2 + 8;          // LDS row stride (halves): 16-byte reads of 16 rows hit distinct banks
+   constexpr int CPR = K2 / 8;         // 16-byte chunks per weight row
+   constexpr int NST = K2 / 16;        // chunks per thread and tile (8 kinds x 16 rows x CPR over 256 threads)
+   __shared__ __attribute__((aligned(16))) half_t sW[2][8][16][RW];
    __shared__ float2 sY16[BSITES];     // exp(16 i b*.r) per site of the block
    // exp(i k.r) of each lane's current row at its 8 sites, lane-private
    __shared__ float2 sAV[4][2][4][64];
@@ -514,28 +515,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
    const int base = a.s0 + (blockIdx.x / a.nparts) * BSITES;
    const int ntile_all = a.nrows16 / 16;
    const int rt0 = part * ntile_all / a.nparts, ntile = (part + 1) * ntile_all / a.nparts;
-   // DMA of row tile rt: instruction j (wave-uniform) fills LDS bytes
-   // [1024 j, 1024 j + 1024) of sW, lane L the 16 bytes at 1024 j + 16 L;
-   // padding chunks take any valid source (never read)
-   // Each wave issues the same number of DMA loads (the surplus ones repeat
-   // the last load, same bytes to the same place), so the wait counts for
-   // the loads issued before them are known at compile time.
-   constexpr int NPW = (NINST + 3) / 4;
-   const int swave = __builtin_amdgcn_readfirstlane(wave);
-   auto wdma = [&](int rt) {
+   // weight staging: thread (row tid / 16, c0 = tid % 16) moves chunks
+   // u = c0 + 16 j of its row's 8 CPR chunks (kind x hi/lo = u / CPR)
+   int soff[NST], doff[NST];
 #pragma unroll
-      for (int jj = 0; jj < NPW; jj++) {
-         const int j = min(swave + 4 * jj, NINST - 1);
-         const int q = 64 * j + lane;
-         const int reg = q / (16 * CR), rem = q - reg * 16 * CR, row = rem / CR, c = rem - row * CR;
-         const int cc = c < K2 / 8 ? c : 0;
-         const half_t *src = a.Wt + ((size_t)reg * a.nrows16 + rt * 16 + row) * K2 + 8 * cc;
-         __builtin_amdgcn_global_load_lds((global_void *)src, (lds_void *)((char *)&sW[0][0][0][0] + 1024 * j),
-                                          16, 0, 0);
-      }
+   for (int j = 0; j < NST; j++) {
+      const int u = (tid & 15) + 16 * j, reg = u / CPR, c = u - reg * CPR, row = tid >> 4;
+      soff[j] = (reg * a.nrows16 + row) * K2 + 8 * c;
+      doff[j] = (reg * 16 + row) * RW + 8 * c;
+   }
+   half8 wst[NST];
+   auto load_w = [&](int rt) {
+      const half_t *src = a.Wt + (size_t)rt * 16 * K2;
+#pragma unroll
+      for (int j = 0; j < NST; j++) wst[j] = *(const half8 *)(src + soff[j]);
    };
-   wdma(rt0);
-   // Z operand [site][Zr | Zi over the padded l range], held for the whole
+   auto store_w = [&](int buf) {
+      half_t *dst = &sW[buf][0][0][0];
+#pragma unroll
+      for (int j = 0; j < NST; j++) *(half8 *)(dst + doff[j]) = wst[j];
+   };
+   load_w(rt0);
+   // Z operand [site][Zr | Zi over l = 0 .. NCOL-1], held for the whole
    // kernel.  Sites past s1 read the last valid row: their products land
    // only in their own output rows, which are never stored.
    half8 zh[2][NB], zl[2][NB];
@@ -565,9 +566,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
    // row metadata of the lane's row, one tile ahead of its use
    int2 hk = a.rowhk[rt0 * 16 + lc];
    float4 kv = a.rowk[rt0 * 16 + lc];
-   asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+   store_w(0);
+   load_w(rt0 + 1 < ntile ? rt0 + 1 : rt0);
+   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    for (int rt = rt0; rt < ntile; rt++) {
+      const int buf = (rt - rt0) & 1;
+      const half_t *W = &sW[buf][0][0][0];
       const int r = rt * 16 + lc;
+      // a lane whose row starts a new (h, k) run takes exp(i h a*.r) and
+      // exp(i |k| b*.r) of its 8 sites from the tables; the loads go out
+      // before the MFMAs and land under them (sites past s1 read the last
+      // valid site; their results are never stored)
+      const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
+      float2 xs[2][4], ys[2][4];
+      if (r < a.nrows && !step16) {
+         const float2 *xrow = a.X + (size_t)hk.x * a.nsp, *yrow = a.Y + (size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp;
+#pragma unroll
+         for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+               const int i = base + 32 * wave + 16 * m + 4 * g + j, ic = i < a.s1 ? i : a.s1 - 1;
+               xs[m][j] = xrow[ic];
+               ys[m][j] = yrow[ic];
+            }
+      }
       floatx4 acc[2][4];
 #pragma unroll
       for (int m = 0; m < 2; m++)
@@ -576,8 +598,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int p = 0; p < 4 * NB; p++) {
          const int ks = p >> 2, kd = p & 3;
-         const half8 bh = *(const half8 *)&sW[kd][0][lc][ks * 32 + 8 * g];
-         const half8 bl = *(const half8 *)&sW[kd][1][lc][ks * 32 + 8 * g];
+         const half8 bh = *(const half8 *)(W + ((2 * kd) * 16 + lc) * RW + ks * 32 + 8 * g);
+         const half8 bl = *(const half8 *)(W + ((2 * kd + 1) * 16 + lc) * RW + ks * 32 + 8 * g);
 #pragma unroll
          for (int m = 0; m < 2; m++) {
             acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
@@ -593,8 +615,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
          __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
          if (p + 2 < 4 * NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
       }
-      // phase state and exp(16 i b*.r) of the lane's 8 sites, read before
-      // the DMA goes out (an LDS access after LDS DMA waits for the DMA)
+      // phase state and exp(16 i b*.r) of the lane's 8 sites
       float2 yv[2][4], sv[2][4];
 #pragma unroll
       for (int m = 0; m < 2; m++)
@@ -603,16 +624,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             yv[m][j] = sAV[wave][m][j][lane];
             sv[m][j] = sY16[32 * wave + 16 * m + 4 * g + j];
          }
-      // every wave is done with this tile's weights: the next tile's DMA
-      // runs under the epilogue (after the last tile it reloads this one,
-      // unread, so that every path issues the same loads)
-      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       const int rnext = rt + 1 < ntile ? rt + 1 : rt;
-      wdma(rnext);
       const int2 hk_n = a.rowhk[rnext * 16 + lc];
       const float4 kv_n = a.rowk[rnext * 16 + lc];
       if (r < a.nrows) {
-         const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
          if (step16) {
             // exp(i (k+16) b*.r) = exp(i k b*.r) exp(i 16 b*.r)
 #pragma unroll
@@ -623,27 +638,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                   yv[m][j] = make_float2(y.x * s.x - y.y * s.y, y.y * s.x + y.x * s.y);
                }
          } else {
-            // a new (h, k) run: exp(i h a*.r) exp(i k b*.r) from the tables,
-            // all loads of an m tile in flight together (sites past s1 read
-            // the last valid site; their results are never stored)
-            const int ak = hk.y < 0 ? -hk.y : hk.y;
+            // a new (h, k) run: exp(i h a*.r) exp(i k b*.r), k < 0 through
+            // the conjugate of exp(i |k| b*.r)
             const float ysg = hk.y < 0 ? -1.f : 1.f;
-            const float2 *xrow = a.X + (size_t)hk.x * a.nsp, *yrow = a.Y + (size_t)ak * a.nsp;
 #pragma unroll
-            for (int m = 0; m < 2; m++) {
-               float2 x[4], y[4];
+            for (int m = 0; m < 2; m++)
 #pragma unroll
                for (int j = 0; j < 4; j++) {
-                  const int i = base + 32 * wave + 16 * m + 4 * g + j, ic = i < a.s1 ? i : a.s1 - 1;
-                  x[j] = xrow[ic];
-                  y[j] = yrow[ic];
+                  const float2 x = xs[m][j], y = ys[m][j];
+                  const float yi = ysg * y.y;
+                  yv[m][j] = make_float2(x.x * y.x - x.y * yi, x.y * y.x + x.x * yi);
                }
-#pragma unroll
-               for (int j = 0; j < 4; j++) {
-                  const float yi = ysg * y[j].y;
-                  yv[m][j] = make_float2(x[j].x * y[j].x - x[j].y * yi, x[j].y * y[j].x + x[j].x * yi);
-               }
-            }
          }
          // P and Q carry power-of-two scales: the sums run in units of the
          // P scale (exact, so the same bits as scaling every term) and the Q
@@ -661,15 +666,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
          cur_h = hk.x;
          cur_k = hk.y;
-      }
-      // the next tile's weights are in LDS for every wave
-      asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-      if (r < a.nrows) {
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) sAV[wave][m][j][lane] = yv[m][j];
       }
+      // tile rt + 1 into the other buffer (last read in tile rt - 1, before
+      // the previous barrier), tile rt + 2 into the registers
+      store_w(buf ^ 1);
+      load_w(rt + 2 < ntile ? rt + 2 : ntile - 1);
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       hk = hk_n;
       kv = kv_n;
    }
