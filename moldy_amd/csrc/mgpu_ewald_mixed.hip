@@ -35,6 +35,8 @@ namespace {
 typedef _Float16 half_t;
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
 
 constexpr double PI_D = 3.14159265358979323846;
 constexpr int KS = 32;        // sites per MFMA k step
@@ -49,6 +51,11 @@ __device__ __forceinline__ void split2(double v, half_t &hi, half_t &lo) {
 __device__ __forceinline__ void split2f(float v, half_t &hi, half_t &lo) {
    hi = (half_t)v;
    lo = (half_t)(v - (float)hi);
+}
+// split2f of two values with packed conversions (the same roundings)
+__device__ __forceinline__ void split2f_pk(float2v v, half2v &hi, half2v &lo) {
+   hi = __builtin_convertvector(v, half2v);
+   lo = __builtin_convertvector(v - __builtin_convertvector(hi, float2v), half2v);
 }
 
 // ------------------------------------------------------------- tables
@@ -169,35 +176,51 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
    const int rr = tid >> 2, sg = tid & 3;
    const int2 hk = shk[rr];
    const bool rvalid = hk.x >= 0;
-   const size_t xoff = rvalid ? (size_t)hk.x * a.nsp : 0;
+   const size_t xoff = rvalid ? (size_t)hk.x * a.nsp : 0;   // row 0 for padding rows
    const size_t yoff = rvalid ? (size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp : 0;
    const float ysgn = hk.y < 0 ? -1.f : 1.f;
-   float4 xr[4], yr[4];
-   auto load_xy = [&](int i0) {
+   // X and Y rows of the thread's 8 sites for two steps: the set one step
+   // ahead feeds the A generation while the other is loaded
+   float4 xr[2][4], yr[2][4];
+   auto load_xy = [&](int i0, int set) {
       const int ib = i0 + 8 * sg;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-         xr[v] = *(const float4 *)(a.X + xoff + ib + 2 * v);
-         yr[v] = *(const float4 *)(a.Y + yoff + ib + 2 * v);
+         xr[set][v] = *(const float4 *)(a.X + xoff + ib + 2 * v);
+         yr[set][v] = *(const float4 *)(a.Y + yoff + ib + 2 * v);
       }
    };
-   auto gen_a = [&](int i0, int buf) {
+   // Rows past nrows (h < 0) read row 0 of the tables: their sums are
+   // never read.  Sites outside the slice's valid range weigh zero; only a
+   // slice's first and last steps have any.
+   auto gen_a = [&](int i0, int buf, int set) {
       const int ib = i0 + 8 * sg;
       const int tlo = sl.z - ib, thi = sl.w - ib;   // valid sites ib + [tlo, thi)
-      half8 arh, arl, aih, ail;
+      float ar[8], ai[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
-         const float4 xv = xr[t >> 1], yv = yr[t >> 1];
+         const float4 xv = xr[set][t >> 1], yv = yr[set][t >> 1];
          const float x0 = (t & 1) ? xv.z : xv.x, x1 = (t & 1) ? xv.w : xv.y;
          const float y0 = (t & 1) ? yv.z : yv.x, y1 = ysgn * ((t & 1) ? yv.w : yv.y);
-         const bool ok = rvalid & (t >= tlo) & (t < thi);
-         const float ar = ok ? x0 * y0 - x1 * y1 : 0.f;
-         const float ai = ok ? x1 * y0 + x0 * y1 : 0.f;
-         half_t h, l;
-         split2f(ar, h, l);
-         arh[t] = h; arl[t] = l;
-         split2f(ai, h, l);
-         aih[t] = h; ail[t] = l;
+         ar[t] = x0 * y0 - x1 * y1;
+         ai[t] = x1 * y0 + x0 * y1;
+      }
+      if (tlo > 0 || thi < 8) {
+#pragma unroll
+         for (int t = 0; t < 8; t++) {
+            const bool ok = (t >= tlo) & (t < thi);
+            ar[t] = ok ? ar[t] : 0.f;
+            ai[t] = ok ? ai[t] : 0.f;
+         }
+      }
+      half8 arh, arl, aih, ail;
+#pragma unroll
+      for (int t = 0; t < 8; t += 2) {
+         half2v h, l;
+         split2f_pk(float2v{ar[t], ar[t + 1]}, h, l);
+         arh[t] = h.x; arh[t + 1] = h.y; arl[t] = l.x; arl[t + 1] = l.y;
+         split2f_pk(float2v{ai[t], ai[t + 1]}, h, l);
+         aih[t] = h.x; aih[t + 1] = h.y; ail[t] = l.x; ail[t + 1] = l.y;
       }
       *(half8 *)&sA[buf][0][rr][8 * sg] = arh;
       *(half8 *)&sA[buf][1][rr][8 * sg] = arl;
@@ -229,14 +252,17 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
    int i0 = sl.x;
    if (i0 < sl.y) {
       load_b(i0);
-      load_xy(i0);
-      gen_a(i0, 0);
+      load_xy(i0, 0);
+      gen_a(i0, 0, 0);
       store_b(0);
-      load_xy(i0 + KS < sl.y ? i0 + KS : i0);
+      load_xy(i0 + KS < sl.y ? i0 + KS : i0, 1);
    }
    __syncthreads();
-   for (int s = 0; i0 < sl.y; i0 += KS, s++) {
-      const int buf = s & 1;
+   // one 32-site step on the A and Z tiles of buffer buf; the X/Y register
+   // set buf ^ 1 (loaded a step ago) feeds the next step's A, set buf takes
+   // the rows two steps ahead.  The loop below runs the steps in pairs, so
+   // both indices are constants and no registers are copied.
+   auto step = [&](const int buf) {
       const int inext = i0 + KS < sl.y ? i0 + KS : i0;   // in bounds
       const int inext2 = i0 + 2 * KS < sl.y ? i0 + 2 * KS : inext;
       half8 ah[4], al8[4];
@@ -246,23 +272,13 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
          al8[m] = *(const half8 *)&sA[buf][2 * wm + 1][64 * rh + m * 16 + lc][8 * g];
       }
       // table loads first: the next step's Z tile, then the X/Y rows two
-      // steps ahead (the A generation below still reads the previous ones)
-      float4 xn[4], yn[4];
-      {
-         load_b(inext);
-         const int ib = inext2 + 8 * sg;
-#pragma unroll
-         for (int v = 0; v < 4; v++) {
-            xn[v] = *(const float4 *)(a.X + xoff + ib + 2 * v);
-            yn[v] = *(const float4 *)(a.Y + yoff + ib + 2 * v);
-         }
-      }
+      // steps ahead
+      load_b(inext);
+      load_xy(inext2, buf);
       // loads stay above; the A generation (VALU, LDS writes) may interleave
       // with the MFMAs below
       __builtin_amdgcn_sched_barrier(0x0606);
-      gen_a(inext, buf ^ 1);   // after the last step: written, never read
-#pragma unroll
-      for (int v = 0; v < 4; v++) { xr[v] = xn[v]; yr[v] = yn[v]; }
+      gen_a(inext, buf ^ 1, buf ^ 1);   // after the last step: written, never read
 #pragma unroll
       for (int n = 0; n < NB; n++) {
          const half8 bh = *(const half8 *)&sB[buf][wn][0][n * 16 + lc][8 * g];
@@ -286,6 +302,13 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
       __builtin_amdgcn_sched_barrier(0);
       store_b(buf ^ 1);
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+   };
+   while (i0 < sl.y) {
+      step(0);
+      i0 += KS;
+      if (i0 >= sl.y) break;
+      step(1);
+      i0 += KS;
    }
    // With RR = Ar Zr, II = Ai Zi, RI = Ar Zi, IR = Ai Zr:
    //   S[+l] = (RR - II) + i (RI + IR),   S[-l] = (RR + II) + i (IR - RI).
