@@ -81,6 +81,19 @@ int  mgpu_set_precision(mgpu_engine *eng, int precision);
  * ewald run one after the other on the CPU. */
 int  mgpu_set_overlap(mgpu_engine *eng, int on);
 
+/* Real-space kernel: 0 automatic (the load-balanced kernel when the grid
+ * holds at least 16 of its waves per CU), 1 the per-offset walk, 2 the
+ * load-balanced kernel (one wave per site over the shell's z-columns of
+ * cells) wherever the shell splits into columns, else the walk.  Both give
+ * the reference's pair set; only the summation order differs.
+ * mgpu_realspace_balanced reports whether the last real-space pass used the
+ * load-balanced kernel.  No counterpart in the reference. */
+#define MGPU_RS_AUTO 0
+#define MGPU_RS_WALK 1
+#define MGPU_RS_BALANCED 2
+int  mgpu_set_realspace_mode(mgpu_engine *eng, int mode);
+int  mgpu_realspace_balanced(const mgpu_engine *eng);
+
 /* ---- the boundary, host buffers (replaces src/force.c:1108 force_calc) ----
  * site/site_force: [3][ld]; c_of_m_frac: [nmols][3] (read when molpbc);
  * chg: [nsites].  Accumulates into site_force, *pe and the upper triangle of

@@ -222,7 +222,7 @@ static __global__ void k_cell_finalize(FinArgs a, const int *__restrict__ c_star
                                        const int *__restrict__ stype, const int *__restrict__ smol,
                                        const int *__restrict__ sfw, double4 *__restrict__ cpos,
                                        float4 *__restrict__ cposf, int2 *__restrict__ cmeta,
-                                       int *__restrict__ corig, int *__restrict__ ccell) {
+                                       int *__restrict__ corig, int *__restrict__ ccell, int *__restrict__ ccz) {
    int c = blockIdx.x * blockDim.x + threadIdx.x;
    if (c >= a.ncells) return;
    int b = c_start[c], n = c_start[c + 1] - b;
@@ -252,6 +252,7 @@ static __global__ void k_cell_finalize(FinArgs a, const int *__restrict__ c_star
          cmeta[pos] = make_int2(stype[is] | (sfw[is] << 8), smol[is]);
          corig[pos] = is;
          ccell[pos] = c;
+         ccz[pos] = iz;
       }
    }
 }
@@ -273,6 +274,7 @@ void mgpu::run_binning(mgpu_engine *e, bool) {
    e->d_cmeta.reserve(e->nsites + 1);
    e->d_corig.reserve(e->nsites + 1);
    e->d_ccell.reserve(e->nsites + 1);
+   e->d_ccz.reserve(e->nsites + 1);
    zero_async(e, e->d_cell_count.p, sizeof(int) * (nc + 1));
    zero_async(e, e->d_cell_nsites.p, sizeof(int) * (nc + 1));
    BinArgs a;
@@ -294,7 +296,7 @@ void mgpu::run_binning(mgpu_engine *e, bool) {
    k_cell_finalize<<<(nc + 255) / 256, 256, 0, e->stream>>>(
        f, e->d_cell_start.p, e->d_cell_entries.p, e->d_cell_site_start.p, e->d_entry_isite.p,
        e->d_entry_num.p, e->d_site.p, e->d_chg_site.p, e->d_site_type.p, e->d_site_mol.p,
-       e->d_site_fw.p, e->d_cpos.p, e->d_cposf.p, e->d_cmeta.p, e->d_corig.p, e->d_ccell.p);
+       e->d_site_fw.p, e->d_cpos.p, e->d_cposf.p, e->d_cmeta.p, e->d_corig.p, e->d_ccell.p, e->d_ccz.p);
 }
 
 // ------------------------------------------------------ site neighbour lists

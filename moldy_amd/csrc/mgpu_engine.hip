@@ -10,6 +10,8 @@
 // (accel.c:293-329) -- are computed once per system.
 #include "mgpu_internal.h"
 
+#include <map>
+
 #include <math.h>
 #include <string.h>
 #include <algorithm>
@@ -232,6 +234,43 @@ void mgpu::set_geometry(mgpu_engine *e, const double h[9]) {
                                     (float)(h[6] * f0 + h[7] * f1 + h[8] * f2), 0.f));
          }
       e->n_full = (int)full.size();
+      // z-columns: for each (dx, dy) the shell's dz must form one interval of
+      // at most nz cells (then a column is at most two contiguous runs of
+      // cell-ordered sites, split at the periodic wrap), |dx| < nx, |dy| < ny,
+      // at most 256 columns and nz below 2048; otherwise rs_ncol = 0 and
+      // the real-space pass walks the offsets one by one
+      {
+         std::map<std::pair<int, int>, std::vector<int>> cz;
+         for (const int4 &o : full) cz[{o.x, o.y}].push_back(o.z);
+         std::vector<int4> cols;
+         std::vector<float4> colD;
+         bool ok = nz < 2048 && cz.size() <= 256;
+         int own = -1;
+         for (auto &kv : cz) {
+            std::vector<int> &z = kv.second;
+            std::sort(z.begin(), z.end());
+            const int lo = z.front(), hi = z.back();
+            if (hi - lo + 1 != (int)z.size() || hi - lo + 1 > nz || lo <= -nz || hi >= nz) ok = false;
+            if (std::abs(kv.first.first) >= nx || std::abs(kv.first.second) >= ny) ok = false;
+            if (kv.first.first == 0 && kv.first.second == 0) own = (int)cols.size();
+            const int dx = kv.first.first, dy = kv.first.second;
+            cols.push_back(make_int4(dx, dy, lo, hi));
+            const double f0 = (double)dx / nx, f1 = (double)dy / ny;
+            colD.push_back(make_float4((float)(h[0] * f0 + h[1] * f1), (float)(h[3] * f0 + h[4] * f1),
+                                       (float)(h[6] * f0 + h[7] * f1), 0.f));
+         }
+         if (own < 0) ok = false;
+         e->rs_ncol = ok ? (int)cols.size() : 0;
+         e->rs_own_col = ok ? own : 0;
+         if (ok) {
+            e->d_rscol.reserve(cols.size() + 1);
+            e->d_rscolD.reserve(cols.size() + 1);
+            HIPCHK(hipMemcpyAsync(e->d_rscol.p, cols.data(), sizeof(int4) * cols.size(), hipMemcpyHostToDevice,
+                                  e->stream));
+            HIPCHK(hipMemcpyAsync(e->d_rscolD.p, colD.data(), sizeof(float4) * colD.size(),
+                                  hipMemcpyHostToDevice, e->stream));
+         }
+      }
       e->d_off.reserve(full.size() + 1);
       e->d_offD.reserve(full.size() + 1);
       HIPCHK(hipMemcpyAsync(e->d_off.p, full.data(), sizeof(int4) * full.size(), hipMemcpyHostToDevice, e->stream));
@@ -690,7 +729,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_mforce); fr(e->d_mtorque); fr(e->d_red); fr(e->d_out); fr(e->d_iflags); fr(e->d_entry_cell);
    fr(e->d_entry_slot); fr(e->d_cell_count); fr(e->d_cell_start); fr(e->d_cell_entries);
    fr(e->d_cell_nsites); fr(e->d_cell_site_start); fr(e->d_scan_tmp); fr(e->d_corig); fr(e->d_cpos);
-   fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_cell_part); fr(e->d_rows);
+   fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_rscol); fr(e->d_rscolD); fr(e->d_ccz); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
    fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_rtmp_side); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
    fr(e->d_adj); fr(e->d_adj_real);
@@ -716,6 +755,17 @@ int mgpu_set_overlap(mgpu_engine *e, int on) {
    }
    return 0;
 }
+
+int mgpu_set_realspace_mode(mgpu_engine *e, int mode) {
+   if (!e || mode < MGPU_RS_AUTO || mode > MGPU_RS_BALANCED) return MGPU_E_ARG;
+   if (e->rs_mode != mode) {
+      e->rs_mode = mode;
+      e->geom_version++;   // captured step graphs follow the kernel choice
+   }
+   return 0;
+}
+
+int mgpu_realspace_balanced(const mgpu_engine *e) { return e ? e->rs_last_bal : 0; }
 
 int mgpu_set_precision(mgpu_engine *e, int p) {
    if (!e || (p != MGPU_FP64 && p != MGPU_MIXED)) return MGPU_E_ARG;

@@ -137,6 +137,15 @@ struct mgpu_engine {
    std::vector<int> off_half;                  // [n_half][3]
    mgpu::DBuf<int4> d_off;                     // full shell: dx,dy,dz,0
    mgpu::DBuf<float4> d_offD;                  // h (dx/nx,dy/ny,dz/nz) as float
+   // the full shell as z-columns of cells (dx, dy, dz lo, dz hi) and their
+   // h (dx/nx, dy/ny, 0) as float, for the load-balanced real-space pass;
+   // rs_ncol = 0 when the shell does not split into columns (see set_geometry)
+   mgpu::DBuf<int4> d_rscol;
+   mgpu::DBuf<float4> d_rscolD;
+   int rs_ncol = 0, rs_own_col = 0;
+   int rs_mode = 0;                             // MGPU_RS_*
+   int rs_last_bal = 0;                         // the last pass used the balanced kernel
+   mgpu::DBuf<int> d_ccz;                      // cell-ordered -> z index of the cell
    mgpu::DBuf<int4> d_off_pl;                  // offsets of an explicit pair-list call
    mgpu::DBuf<int> d_pl_count, d_pl_start;     // pair-list CSR counts / starts
    mgpu::DBuf<int> d_pl_nab, d_pl_pbc, d_pl_nfnab;   // site neighbour lists (CSR)

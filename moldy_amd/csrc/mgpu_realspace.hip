@@ -39,6 +39,12 @@ struct RSArgs {
    const int *corig;
    const int *ccell;
    int nsites;
+   // z-columns of the shell for the load-balanced pass
+   int ncol, own_col;
+   const int4 *cols;      // dx, dy, dz lo, dz hi
+   const float4 *colD;    // h (dx/nx, dy/ny, 0)
+   const int *ccz;        // z index of each cell-ordered site's cell
+   float cvx, cvy, cvz;   // h (0, 0, 1/nz)
    const double *potpar;
    double *force;
    double *part;
@@ -202,28 +208,223 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
    }
 }
 
+// Load-balanced variant: one wave per site.  The full shell is cut into
+// z-columns of cells; a column's cells are consecutive in cell order, so
+// its sites form one run of the cell-ordered arrays (two when it wraps
+// through the periodic boundary in z).  The lanes build the site's runs
+// ("pieces", at most two per column), a wave prefix sum gives each piece's
+// first candidate, and the candidates are then dealt to the lanes 64 at a
+// time, each lane finding its piece by binary search over the 64 pieces
+// that can meet the batch.  Every lane does a pair per step whatever the
+// cell occupancies, where the per-offset walk above waits on the fullest
+// cell of the wave at every offset.  The pair set and the pair term are the
+// walk's; only the summation order differs.
+constexpr int RSB_MAXP = 512;   // pieces per site: two per column, at most 256 columns
+
+template <class R, int PT, bool COUL>
+__global__ __launch_bounds__(256) void k_realspace_bal(RSArgs a) {
+   __shared__ R spot[MGPU_MAX_ID * MGPU_MAX_ID * MGPU_NPOTP];
+   __shared__ double sreloc[81];
+   __shared__ int sst[4][RSB_MAXP + 1];   // first candidate of each piece, then the total
+   __shared__ int2 spd[4][RSB_MAXP];      // (site index - first candidate, column | z field << 10)
+   const int npp = a.max_id * a.max_id * MGPU_NPOTP;
+   for (int t = threadIdx.x; t < npp; t += blockDim.x) spot[t] = (R)a.potpar[t];
+   for (int t = threadIdx.x; t < 81; t += blockDim.x) sreloc[t] = a.reloc[t];
+   __syncthreads();
+
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   const int p = blockIdx.x * 4 + wave;
+   double *pp = a.part + 8 * (size_t)p;
+   if (p >= a.nsites) {   // wave-uniform; no block barrier follows
+      if (lane < 8) pp[lane] = 0.0;
+      return;
+   }
+   const int cell = a.ccell[p];
+   const int ix = cell / (a.ny * a.nz), iy = cell / a.nz - a.ny * ix, iz = cell - a.nz * (iy + a.ny * ix);
+   R xi, yi, zi, qi;
+   Pos<R>::load(a, p, xi, yi, zi, qi);
+   const int2 mi2 = a.cmeta[p];
+   const int ti = mi2.x & 0xff, fwi = mi2.x >> 8, mi = mi2.y;
+   const R *prow = &spot[ti * a.max_id * MGPU_NPOTP];
+   const R norm = (R)a.norm, alpha = (R)a.alpha;
+
+   // pieces of columns lane, lane + 64, ...: (first site, count, column,
+   // z field).  The z field is, in FP32, the offset that turns a site's
+   // cell z index into dz (+ 4096); in FP64 the periodic image code.
+   int pj[8], pn[8], pz[8];
+#pragma unroll
+   for (int m = 0; m < 4; m++) {
+      pn[2 * m] = pn[2 * m + 1] = 0;
+      pj[2 * m] = pj[2 * m + 1] = 0;
+      pz[2 * m] = pz[2 * m + 1] = 0;
+      const int c = lane + 64 * m;
+      if (c < a.ncol) {
+         const int4 col = a.cols[c];
+         const int ex = ix + col.x, ey = iy + col.y;
+         const int sx = ex < 0 ? -1 : (ex >= a.nx ? 1 : 0), sy = ey < 0 ? -1 : (ey >= a.ny ? 1 : 0);
+         const int cb = a.nz * ((ey - sy * a.ny) + a.ny * (ex - sx * a.nx));
+         const int ez0 = iz + col.z, ez1 = iz + col.w;
+         const int s0 = ez0 < 0 ? -1 : (ez0 >= a.nz ? 1 : 0), s1 = ez1 < 0 ? -1 : (ez1 >= a.nz ? 1 : 0);
+         const int za = ez0 - s0 * a.nz, zb = s1 == s0 ? ez1 - s0 * a.nz : a.nz - 1;
+         const int j0 = a.css[cb + za], j1 = a.css[cb + zb + 1];
+         pj[2 * m] = j0;
+         pn[2 * m] = j1 - j0;
+         if (s1 != s0) {
+            const int zc = ez1 - s1 * a.nz, k0 = a.css[cb], k1 = a.css[cb + zc + 1];
+            pj[2 * m + 1] = k0;
+            pn[2 * m + 1] = k1 - k0;
+         }
+         if (sizeof(R) == 4) {
+            pz[2 * m] = c | ((s0 * a.nz - iz + 4096) << 10);
+            pz[2 * m + 1] = c | ((s1 * a.nz - iz + 4096) << 10);
+         } else {
+            pz[2 * m] = c | ((9 * (sx + 1) + 3 * (sy + 1) + (s0 + 1)) << 10);
+            pz[2 * m + 1] = c | ((9 * (sx + 1) + 3 * (sy + 1) + (s1 + 1)) << 10);
+         }
+      }
+   }
+   // non-empty pieces, packed in lane order: count and candidates per lane,
+   // exclusive prefix over the wave
+   int cnt = 0, tot = 0;
+#pragma unroll
+   for (int q = 0; q < 8; q++) {
+      cnt += pn[q] > 0;
+      tot += pn[q];
+   }
+   int icnt = cnt, itot = tot;
+   for (int d = 1; d < 64; d <<= 1) {
+      const int uc = __shfl_up(icnt, d, 64), ut = __shfl_up(itot, d, 64);
+      if (lane >= d) {
+         icnt += uc;
+         itot += ut;
+      }
+   }
+   const int np = __shfl(icnt, 63, 64), total = __shfl(itot, 63, 64);
+   {
+      int k = icnt - cnt, st = itot - tot;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+         if (pn[q] > 0) {
+            sst[wave][k] = st;
+            spd[wave][k] = make_int2(pj[q] - st, pz[q]);
+            k++;
+            st += pn[q];
+         }
+      if (lane == 63) sst[wave][np] = total;
+   }
+   // the wave's own LDS writes before its reads
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+   R fx = 0, fy = 0, fz = 0, ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
+   int kf = 0;   // the piece holding the batch's first candidate
+   for (int base = 0; base < total; base += 64) {
+      const int c = base + lane;
+      // last piece k in [kf, kf + 64) with sst[k] <= c (each piece holds a
+      // candidate, so the 64 candidates of the batch lie in that range)
+      const int kend = min(kf + 64, np);
+      int k = kf;
+#pragma unroll
+      for (int s = 32; s >= 1; s >>= 1)
+         if (k + s < kend && sst[wave][k + s] <= c) k += s;
+      kf = __shfl(k, 63, 64);
+      const bool act = c < total;
+      const int2 pd = spd[wave][k];
+      const int j = act ? c + pd.x : p;
+      const int2 mj = a.cmeta[j];
+      R xj, yj, zj, qj;
+      Pos<R>::load(a, j, xj, yj, zj, qj);
+      const int colx = pd.y & 1023, zf = pd.y >> 10;
+      R shx, shy, shz;
+      bool self;
+      if (sizeof(R) == 4) {
+         const int dz = a.ccz[j] + zf - 4096;
+         const float4 D = a.colD[colx];
+         shx = (R)(D.x + (float)dz * a.cvx);
+         shy = (R)(D.y + (float)dz * a.cvy);
+         shz = (R)(D.z + (float)dz * a.cvz);
+         self = colx == a.own_col && dz == 0;
+      } else {
+         shx = (R)sreloc[3 * zf];
+         shy = (R)sreloc[3 * zf + 1];
+         shz = (R)sreloc[3 * zf + 2];
+         self = colx == a.own_col && zf == 13;
+      }
+      const int tj = mj.x & 0xff, fwj = mj.x >> 8;
+      const bool skip = !act || (self && j == p) || (fwi & fwj);
+      R rx = xj - xi + shx, ry = yj - yi + shy, rz = zj - zi + shz;
+      R rsq = rx * rx + ry * ry + rz * rz;
+      if (!skip && rsq < R(0.25) && mi != mj.y) {
+         int io = a.corig[p], jo = a.corig[j];
+         if (io < jo) {
+            int slot = atomicAdd(a.tooclose, 1);
+            if (slot < 16) {
+               a.tooclose[1 + 2 * slot] = io;
+               a.tooclose[2 + 2 * slot] = jo;
+            }
+         }
+      }
+      if (a.strict_sub && rsq > (R)a.cutsq) rsq = (R)a.cut100sq;
+      if (skip) rsq = R(1);   // a finite stand-in; its terms are dropped below
+      R u, f;
+      pair_term<R, PT, COUL>(rsq, qi * qj, norm, alpha, prow + tj * MGPU_NPOTP, u, f);
+      if (sizeof(R) == 4 && a.skip_intra_u && mi == mj.y && rsq < (R)a.half_box_sq) u = 0;
+      if (skip) {
+         u = 0;
+         f = 0;
+      }
+      fx -= f * rx;
+      fy -= f * ry;
+      fz -= f * rz;
+      ep += u;
+      w00 += f * rx * rx; w01 += f * rx * ry; w02 += f * rx * rz;
+      w11 += f * ry * ry; w12 += f * ry * rz; w22 += f * rz * rz;
+   }
+   const double gx = wave_sum((double)fx), gy = wave_sum((double)fy), gz = wave_sum((double)fz);
+   double v[7] = {(double)ep, (double)w00, (double)w01, (double)w02, (double)w11, (double)w12, (double)w22};
+#pragma unroll
+   for (int q = 0; q < 7; q++) v[q] = wave_sum(v[q]);
+   if (lane == 0) {
+      const int io = a.corig[p];
+      a.force[io] += gx;
+      a.force[a.nsp + io] += gy;
+      a.force[2 * a.nsp + io] += gz;
+      for (int q = 0; q < 7; q++) pp[q] = 0.5 * v[q];
+      pp[7] = 0;
+   }
+}
+
 template <class R, bool COUL>
-void launch_pt(int pt, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
+void launch_pt(int pt, bool bal, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
+#define RS_CASE(n)                                                           \
+   case n:                                                                   \
+      if (bal) k_realspace_bal<R, n, COUL><<<g, b, 0, s>>>(a);               \
+      else k_realspace<R, n, COUL><<<g, b, 0, s>>>(a);                       \
+      break;
    switch (pt) {
-   case 0: k_realspace<R, 0, COUL><<<g, b, 0, s>>>(a); break;
-   case 1: k_realspace<R, 1, COUL><<<g, b, 0, s>>>(a); break;
-   case 2: k_realspace<R, 2, COUL><<<g, b, 0, s>>>(a); break;
-   case 3: k_realspace<R, 3, COUL><<<g, b, 0, s>>>(a); break;
-   case 4: k_realspace<R, 4, COUL><<<g, b, 0, s>>>(a); break;
-   case 6: k_realspace<R, 6, COUL><<<g, b, 0, s>>>(a); break;
+      RS_CASE(0) RS_CASE(1) RS_CASE(2) RS_CASE(3) RS_CASE(4) RS_CASE(6)
    default: throw StatusError{MGPU_E_PTYPE, "unknown potential type"};
    }
+#undef RS_CASE
 }
 
 }  // namespace
 
 void mgpu::run_realspace(mgpu_engine *e) {
    int nc = e->ncells;
-   // shell parts per site: at least 4, more (up to 256, four waves of one
-   // 256-thread block) until the grid holds about 16 waves per CU
+   // one wave per site (load-balanced over the shell's z-columns) when that
+   // puts at least 16 waves on every CU; otherwise, or when the shell does
+   // not split into columns, shell parts per site: at least 4, more (up to
+   // 256, four waves of one 256-thread block) until the grid holds about 16
+   // waves per CU
+   const bool bal = e->rs_ncol > 0 && (e->rs_mode == MGPU_RS_BALANCED ||
+                                       (e->rs_mode == MGPU_RS_AUTO && (long)e->nsites >= 16L * e->ncu));
+   e->rs_last_bal = bal;
    int lognq = 2;
    while (lognq < 8 && (long)e->nsites << lognq < 16L * 64 * e->ncu) lognq++;
-   const int nthreads = e->nsites << lognq, nblk = (nthreads + 255) / 256, nwaves = nblk * 4;
+   const long nthreads = bal ? 64L * e->nsites : (long)e->nsites << lognq;
+   const int nblk = (int)((nthreads + 255) / 256), nwaves = nblk * 4;
    e->d_cell_part.reserve((size_t)nwaves * 8 + 8);
    RSArgs a;
    a.ncells = nc; a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.nfull = e->n_full;
@@ -254,6 +455,9 @@ void mgpu::run_realspace(mgpu_engine *e) {
    a.corig = e->d_corig.p;
    a.ccell = e->d_ccell.p;
    a.nsites = e->nsites;
+   a.ncol = e->rs_ncol; a.own_col = e->rs_own_col;
+   a.cols = e->d_rscol.p; a.colD = e->d_rscolD.p; a.ccz = e->d_ccz.p;
+   a.cvx = (float)(e->h[2] / e->nz); a.cvy = (float)(e->h[5] / e->nz); a.cvz = (float)(e->h[8] / e->nz);
    a.potpar = e->d_potpar.p;
    a.force = e->d_force_site.p;
    a.part = e->d_cell_part.p;
@@ -263,11 +467,11 @@ void mgpu::run_realspace(mgpu_engine *e) {
    {
       ProfScope ps(e, "realspace");
       if (e->precision == MGPU_FP64) {
-         if (coul) launch_pt<double, true>(e->ptype, g, b, e->stream, a);
-         else launch_pt<double, false>(e->ptype, g, b, e->stream, a);
+         if (coul) launch_pt<double, true>(e->ptype, bal, g, b, e->stream, a);
+         else launch_pt<double, false>(e->ptype, bal, g, b, e->stream, a);
       } else {
-         if (coul) launch_pt<float, true>(e->ptype, g, b, e->stream, a);
-         else launch_pt<float, false>(e->ptype, g, b, e->stream, a);
+         if (coul) launch_pt<float, true>(e->ptype, bal, g, b, e->stream, a);
+         else launch_pt<float, false>(e->ptype, bal, g, b, e->stream, a);
       }
    }
    e->d_out.reserve(OUT_SIZE);

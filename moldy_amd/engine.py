@@ -211,6 +211,17 @@ class ForceEngine:
         """Real-space pass on a second stream beside the Ewald sum."""
         self._chk(lib().mgpu_set_overlap(self.h_, int(bool(on))))
 
+    RS_AUTO, RS_WALK, RS_BALANCED = 0, 1, 2
+
+    def set_realspace_mode(self, mode):
+        """Real-space kernel: RS_AUTO, RS_WALK (per-offset walk) or
+        RS_BALANCED (one wave per site over the shell's z-columns)."""
+        self._chk(lib().mgpu_set_realspace_mode(self.h_, int(mode)))
+
+    def realspace_balanced(self):
+        """Whether the last real-space pass used the load-balanced kernel."""
+        return bool(lib().mgpu_realspace_balanced(self.h_))
+
     # ------------------------------------------------------- the boundary
     def force_calc(self, h, xyz, fxyz=None, chg=None, c_of_m=None):
         n = self.sys.nsites

@@ -110,6 +110,50 @@ def test_force_calc_fp64(eng_mod, case):
     e.close()
 
 
+# systems whose shell splits into z-columns of cells (the clay's shell does
+# not, and falls back to the per-offset walk in either mode)
+BALANCED_CASES = tuple(c for c in CASES if c != "clay")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("case", list(CASES))
+def test_force_calc_fp64_both_kernels(eng_mod, case, mode):
+    """The per-offset walk (1) and the load-balanced kernel (2) each against
+    the oracle: same pair set, forces, energy, virial and close-pair count."""
+    s = CASES[case]()
+    o = _oracle(s)
+    xyz = o.make_sites()
+    ref = o.force_calc(xyz)
+    e = eng_mod.ForceEngine(s)
+    e.set_realspace_mode(mode)
+    got = e.force_calc(s.h, xyz)
+    assert e.realspace_balanced() == (mode == 2 and case in BALANCED_CASES)
+    assert rel_err(got["force"], ref["force"]) < TOL64
+    assert abs(got["pe"] - ref["pe"]) <= TOL64 * max(abs(ref["pe"]), 1e-300)
+    assert rel_err(np.triu(got["stress"]), np.triu(ref["stress"])) < TOL64
+    assert got["info"].n_too_close == ref["info"].n_too_close
+    e.close()
+
+
+@pytest.mark.parametrize("case", BALANCED_CASES)
+def test_mixed_realspace_kernels_agree(eng_mod, case):
+    """In FP32 the two real-space kernels differ only in summation order."""
+    s = CASES[case]()
+    out = []
+    for mode in (1, 2):
+        e = eng_mod.ForceEngine(s, precision=1)
+        e.set_realspace_mode(mode)
+        e.upload_state()
+        pe, dip, st = e.eval_forces_device()
+        assert e.realspace_balanced() == (mode == 2)
+        out.append((pe.copy(), st.copy(), e.download()[3].copy()))
+        e.close()
+    (pa, sa, fa), (pb, sb, fb) = out
+    assert rel_err(fb, fa) < 1e-6
+    assert rel_err(sb, sa) < 1e-6
+    assert np.max(np.abs(pb - pa)) <= 1e-6 * np.max(np.abs(pa))
+
+
 @pytest.mark.parametrize("case", [c for c in CASES if not c.startswith("argon")])
 def test_ewald_fp64(eng_mod, case):
     s = CASES[case]()
@@ -193,15 +237,17 @@ def test_force_calc_accumulates(eng_mod):
     e.close()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("case", ["water", "salt", "silica", "mgclh2o", "argon"])
-def test_overlapped_real_space_is_bitwise_serial(eng_mod, case, precision):
+def test_overlapped_real_space_is_bitwise_serial(eng_mod, case, precision, mode):
     """The real-space pass on its own stream beside the Ewald sum sums every
     site force in the serial order: identical results either way."""
     s = CASES[case]()
     out = []
     for overlap in (False, True):
         e = eng_mod.ForceEngine(s, precision=precision)
+        e.set_realspace_mode(mode)
         e.set_overlap(overlap)
         e.upload_state()
         for _ in range(2):
