@@ -731,7 +731,7 @@ void mgpu_destroy(mgpu_engine *e) {
    fr(e->d_cell_nsites); fr(e->d_cell_site_start); fr(e->d_scan_tmp); fr(e->d_corig); fr(e->d_cpos);
    fr(e->d_cposf); fr(e->d_cmeta); fr(e->d_off); fr(e->d_offD); fr(e->d_rscol); fr(e->d_rscolD); fr(e->d_ccz); fr(e->d_cell_part); fr(e->d_rows);
    fr(e->d_krow); fr(e->d_tx); fr(e->d_ty); fr(e->d_tz); fr(e->d_sk); fr(e->d_wk); fr(e->d_mix);
-   fr(e->d_rpart); fr(e->d_fpart); fr(e->d_rtmp); fr(e->d_rtmp_side); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
+   fr(e->d_rpart); fr(e->d_fpart); fr(e->d_zt64); fr(e->d_w64); fr(e->d_rtmp); fr(e->d_rtmp_side); fr(e->d_off_pl); fr(e->d_pl_count); fr(e->d_pl_start);
    fr(e->d_adj); fr(e->d_adj_real);
    fr(e->d_pl_nab); fr(e->d_pl_pbc); fr(e->d_pl_nfnab); fr(e->d_rdf_blk); fr(e->d_rdf_hist); fr(e->d_rdf_bins);
    fr(e->d_mom); fr(e->d_amom); fr(e->d_scal); fr(e->d_trace); fr(e->d_spec_dyn);

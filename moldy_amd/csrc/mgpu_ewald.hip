@@ -307,6 +307,190 @@ __global__ __launch_bounds__(256) void k_ewald_force(EfArgs a) {
    fp[2 * a.nsp + i] = fz;
 }
 
+// ---------------- back-projection on the FP64 matrix cores
+// With P[r,i] = sum_l Z[l,i] conj W[r,l] and Q[r,i] = sum_l l Z[l,i] conj W[r,l]
+// over the row's run of l (ewald.c:557-566), the force is
+//   F_i = sum_r Im(A[r,i] ((kx, ky, kzt)(r) P[r,i] + (0, 0, c2) Q[r,i])),
+// A[r,i] = exp(i(h a* + k b*).r_i).  As Z[-l] = conj Z[l], the columns are
+// l >= 0 only: the K = 2 NC real columns are Zr[0..NC) then Zi[0..NC), and the
+// weights of W[r,+l] and W[r,-l] fold onto them (four kinds: Re P, Im P,
+// Re Q, Im Q).  v_mfma_f64_16x16x4f64 with the weights as the A operand
+// (row = lane % 16, column 4s + lane / 16) and Z as the B operand (site =
+// lane % 16) leaves rows lane/16 + 4v of one site in item v, so each lane
+// sums the forces of one site.
+
+// Z operand, site-major: row i = Zr[0..NC) then Zi[0..NC) of site i (zeros
+// past lmax and past ns), one 64-site tile per block through LDS.
+__global__ __launch_bounds__(64) void k_zt64(int ns, int nsp, int lmax, int nc, const double2 *__restrict__ tz,
+                                             double *__restrict__ zt) {
+   extern __shared__ double szt[];   // [64][2 nc + 1]
+   const int K2 = 2 * nc, RS = K2 + 1, base = blockIdx.x * 64, t = threadIdx.x, i = base + t;
+   for (int l = 0; l < nc; l++) {
+      double2 z = make_double2(0, 0);
+      if (l <= lmax && i < ns) z = tz[(size_t)l * nsp + i];
+      szt[t * RS + l] = z.x;
+      szt[t * RS + nc + l] = z.y;
+   }
+   __syncthreads();
+   for (int q = t; q < 64 * K2; q += 64) {
+      const int row = q / K2, c = q - row * K2;
+      zt[(size_t)(base + row) * K2 + c] = szt[row * RS + c];
+   }
+}
+
+// Folded weight operands, lane-ordered: element [kind][tile][s][g][lc] is
+// kind's weight of row 16 tile + lc at column 4 s + g.  wo selects the
+// per-k weights (Wc, Ws) or, for framework sites, (Wcn, Wsn).
+__global__ void k_w64(int nrows, int nrows16, int nc, int ksteps, int wo, const KRow *__restrict__ rows,
+                      const double *__restrict__ w, double *__restrict__ wt) {
+   const int K2 = 2 * nc;
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= nrows16 * K2) return;
+   const int r = idx / K2, kk = idx - r * K2;
+   const int part = kk >= nc, l = kk - part * nc;
+   double wpr = 0, wpi = 0, wmr = 0, wmi = 0;
+   if (r < nrows) {
+      const KRow R = rows[r];
+      if (l >= R.l0 && l < R.l0 + R.nl) {
+         const double *q = w + 4 * (size_t)(R.kstart + l - R.l0) + wo;
+         wpr = q[0];
+         wpi = q[1];
+      }
+      if (l > 0 && -l >= R.l0 && -l < R.l0 + R.nl) {
+         const double *q = w + 4 * (size_t)(R.kstart - l - R.l0) + wo;
+         wmr = q[0];
+         wmi = q[1];
+      }
+   }
+   double v[4];
+   if (!part) { v[0] = wpr + wmr; v[1] = -(wpi + wmi); v[2] = l * (wpr - wmr); v[3] = l * (wmi - wpi); }
+   else { v[0] = wpi - wmi; v[1] = wpr - wmr; v[2] = l * (wpi + wmi); v[3] = l * (wpr + wmr); }
+   const int tile = r >> 4, lc = r & 15, s = kk >> 2, g = kk & 3;
+   const size_t ntile = (size_t)nrows16 / 16;
+#pragma unroll
+   for (int kind = 0; kind < 4; kind++)
+      wt[((kind * ntile + tile) * ksteps + s) * 64 + g * 16 + lc] = v[kind];
+}
+
+struct Ef64Args {
+   int s0, s1, nsp, nrows, nrows16, kmax, nparts;
+   double c2;
+   const KRow *rows;
+   const double *zt;      // [nsp][4 KST]
+   const double *wt;      // [4][ntile][KST][64]
+   const double2 *tx, *ty;
+   double *fpart;         // [nparts][3][nsp]
+};
+
+// Four waves of 32 sites (two 16-site tiles) per workgroup, the row tiles of
+// one partition (blockIdx.x % nparts, consecutive workgroups on different
+// partitions so that each XCD's L2 keeps one partition's weights).  Per row
+// tile: 8 KST MFMAs, then the epilogue, where each lane advances
+// A = exp(i(h a* + k b*).r) of its four rows at its site by exp(16 i b*.r)
+// along a run of one h, or takes it from the tables at a new run.
+template <int KST>
+__global__ __launch_bounds__(256) void k_force64(Ef64Args a) {
+   constexpr int K2 = 4 * KST;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
+   const int part = blockIdx.x % a.nparts;
+   const int base = a.s0 + (blockIdx.x / a.nparts) * 128 + 32 * wave;
+   const int ntile_all = a.nrows16 / 16;
+   const int rt0 = part * ntile_all / a.nparts, rt1 = (part + 1) * ntile_all / a.nparts;
+   int site[2];
+   double zb[2][KST];
+   double2 y16[2];
+#pragma unroll
+   for (int m = 0; m < 2; m++) {
+      const int i = base + 16 * m + lc;
+      site[m] = i < a.s1 ? i : a.s1 - 1;   // past s1: the last site again, never stored
+      const double *zr = a.zt + (size_t)site[m] * K2 + g;
+#pragma unroll
+      for (int s = 0; s < KST; s++) zb[m][s] = zr[4 * s];
+      y16[m] = a.kmax >= 16 ? a.ty[(size_t)16 * a.nsp + site[m]] : make_double2(1, 0);
+   }
+   double f[2][3] = {{0, 0, 0}, {0, 0, 0}};
+   double2 ph[2][4];
+   int cur_h[4], cur_k[4];
+#pragma unroll
+   for (int v = 0; v < 4; v++) {
+      cur_h[v] = -1;
+      cur_k[v] = 0;
+      ph[0][v] = ph[1][v] = make_double2(1, 0);
+   }
+   const size_t kstride = (size_t)ntile_all * KST * 64;
+   for (int rt = rt0; rt < rt1; rt++) {
+      dbl4 acc[2][4];
+#pragma unroll
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+         for (int kd = 0; kd < 4; kd++) acc[m][kd] = dbl4{0, 0, 0, 0};
+      const double *wb = a.wt + (size_t)rt * KST * 64 + lane;
+#pragma unroll
+      for (int s = 0; s < KST; s++) {
+         double wv[4];
+#pragma unroll
+         for (int kd = 0; kd < 4; kd++) wv[kd] = wb[kd * kstride + s * 64];
+#pragma unroll
+         for (int kd = 0; kd < 4; kd++)
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+               acc[m][kd] = __builtin_amdgcn_mfma_f64_16x16x4f64(wv[kd], zb[m][s], acc[m][kd], 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+         const int r = rt * 16 + g + 4 * v;
+         if (r >= a.nrows) continue;
+         const KRow R = a.rows[r];
+         if (R.h == cur_h[v] && R.k == cur_k[v] + 16) {
+#pragma unroll
+            for (int m = 0; m < 2; m++) ph[m][v] = cmul(ph[m][v], y16[m]);
+         } else {
+            const int ak = R.k < 0 ? -R.k : R.k;
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+               const double2 x = a.tx[(size_t)R.h * a.nsp + site[m]];
+               double2 y = a.ty[(size_t)ak * a.nsp + site[m]];
+               if (R.k < 0) y.y = -y.y;
+               ph[m][v] = cmul(x, y);
+            }
+         }
+         cur_h[v] = R.h;
+         cur_k[v] = R.k;
+#pragma unroll
+         for (int m = 0; m < 2; m++) {
+            const double Ar = ph[m][v].x, Ai = ph[m][v].y;
+            const double t1 = Ar * acc[m][1][v] + Ai * acc[m][0][v];
+            const double t2 = Ar * acc[m][3][v] + Ai * acc[m][2][v];
+            f[m][0] += R.kx * t1;
+            f[m][1] += R.ky * t1;
+            f[m][2] += R.kzt * t1 + a.c2 * t2;
+         }
+      }
+   }
+   // the four lanes of a site (g = 0..3) hold rows g + 4v: sum them
+#pragma unroll
+   for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+         double t = f[m][c];
+         t += __shfl_xor(t, 16, 64);
+         t += __shfl_xor(t, 32, 64);
+         f[m][c] = t;
+      }
+   if (g == 0) {
+#pragma unroll
+      for (int m = 0; m < 2; m++) {
+         const int i = base + 16 * m + lc;
+         if (i < a.s1) {
+            double *fp = a.fpart + (size_t)part * 3 * a.nsp;
+            fp[i] = f[m][0];
+            fp[a.nsp + i] = f[m][1];
+            fp[2 * a.nsp + i] = f[m][2];
+         }
+      }
+   }
+}
+
 __global__ void k_add_parts(int ns, int nsp, int npart, const double *__restrict__ fpart,
                             double *__restrict__ force) {
    int i = blockIdx.x * 256 + threadIdx.x;
@@ -386,6 +570,15 @@ static int sfac_partials(mgpu_engine *e, int nsxf) {
    return nch;
 }
 
+// Column steps of the matrix-core back-projection for this lmax (2 KST >=
+// lmax + 1 columns of each of Zr and Zi), or 0 past lmax 47, where the
+// per-site walk runs instead.
+static int force64_ksteps(int lmax) {
+   for (int kst : {8, 12, 16, 20, 24})
+      if (2 * kst >= lmax + 1) return kst;
+   return 0;
+}
+
 // Arguments of the back-projection over the current trig tables, weights
 // d_wk and partials d_fpart; sites from nsxf on use the framework weights
 static EfArgs force_args(mgpu_engine *e, int nsxf) {
@@ -399,6 +592,58 @@ static EfArgs force_args(mgpu_engine *e, int nsxf) {
    ef.ty1 = e->d_ty.p + (e->kmax > 0 ? nsp : 0);
    ef.fpart = e->d_fpart.p;
    return ef;
+}
+
+// out[3][nsp] += the back-projection of the weights d_wk over the current
+// trig tables (sites from nsxf on with the framework weights), summed over
+// the row partitions in partition order; join_side before the sum when
+// asked (the real-space pass on the side stream adds into the same forces).
+static void backproject_fp64(mgpu_engine *e, int nsxf, double *out, bool join) {
+   const int ns = e->nsites, nsp = e->nsp;
+   const int kst = force64_ksteps(e->lmax);
+   if (kst == 0) {
+      const int npart = e->ew_npart;
+      e->d_fpart.reserve((size_t)npart * 3 * nsp);
+      const EfArgs ef = force_args(e, nsxf);
+      dim3 g((ns + 255) / 256, npart);
+      k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
+      if (join) join_side(e);
+      k_add_parts<<<(ns + 255) / 256, 256, 0, e->stream>>>(ns, nsp, npart, e->d_fpart.p, out);
+      return;
+   }
+   const int nc = 2 * kst, K2 = 4 * kst;
+   const int nrows16 = (e->nrows + 15) / 16 * 16, ntile = nrows16 / 16;
+   const int nparts = std::max(1, std::min(e->ew_npart, ntile));
+   e->d_fpart.reserve((size_t)nparts * 3 * nsp);
+   e->d_zt64.reserve((size_t)nsp * K2);
+   k_zt64<<<nsp / 64, 64, sizeof(double) * 64 * (K2 + 1), e->stream>>>(ns, nsp, e->lmax, nc, e->d_tz.p,
+                                                                       e->d_zt64.p);
+   const size_t wsz = (size_t)4 * nrows16 * K2;
+   const bool fw = nsxf < ns;
+   e->d_w64.reserve(fw ? 2 * wsz : wsz);
+   const int nw = (nrows16 * K2 + 255) / 256;
+   k_w64<<<nw, 256, 0, e->stream>>>(e->nrows, nrows16, nc, kst, 0, e->d_rows.p, e->d_wk.p, e->d_w64.p);
+   if (fw) k_w64<<<nw, 256, 0, e->stream>>>(e->nrows, nrows16, nc, kst, 2, e->d_rows.p, e->d_wk.p, e->d_w64.p + wsz);
+   Ef64Args a;
+   a.nsp = nsp; a.nrows = e->nrows; a.nrows16 = nrows16; a.kmax = e->kmax; a.nparts = nparts;
+   a.c2 = 2 * PI_D * e->hinv[8];
+   a.rows = e->d_rows.p; a.zt = e->d_zt64.p; a.tx = e->d_tx.p; a.ty = e->d_ty.p; a.fpart = e->d_fpart.p;
+   auto run = [&](int s0, int s1, const double *wt) {
+      if (s1 <= s0) return;
+      a.s0 = s0; a.s1 = s1; a.wt = wt;
+      const dim3 g((unsigned)(((s1 - s0 + 127) / 128) * nparts));
+      switch (kst) {
+      case 8: k_force64<8><<<g, 256, 0, e->stream>>>(a); break;
+      case 12: k_force64<12><<<g, 256, 0, e->stream>>>(a); break;
+      case 16: k_force64<16><<<g, 256, 0, e->stream>>>(a); break;
+      case 20: k_force64<20><<<g, 256, 0, e->stream>>>(a); break;
+      default: k_force64<24><<<g, 256, 0, e->stream>>>(a); break;
+      }
+   };
+   run(0, nsxf, e->d_w64.p);
+   run(nsxf, ns, fw ? e->d_w64.p + wsz : e->d_w64.p);
+   if (join) join_side(e);
+   k_add_parts<<<(ns + 255) / 256, 256, 0, e->stream>>>(ns, nsp, nparts, e->d_fpart.p, out);
 }
 
 void mgpu::run_ewald_fp64(mgpu_engine *e) {
@@ -426,16 +671,9 @@ void mgpu::run_ewald_fp64(mgpu_engine *e) {
    }
    e->d_out.reserve(OUT_SIZE);
    reduce_cols(e, e->d_red.p, nkb, 8, e->d_out.p + OUT_RECIP);
-   const int npart = e->ew_npart;
-   e->d_fpart.reserve((size_t)npart * 3 * nsp);
-   const EfArgs ef = force_args(e, e->nsitesxf);
    {
       ProfScope ps(e, "ewald_force");
-      dim3 g((e->nsites + 255) / 256, npart);
-      k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
-      join_side(e);
-      k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p,
-                                                                  e->d_force_site.p);
+      backproject_fp64(e, e->nsitesxf, e->d_force_site.p, true);
    }
 }
 
@@ -605,12 +843,7 @@ static void backproject(mgpu_engine *e, const double *w, const double4 *kc, cons
    e->d_wk.reserve((size_t)e->nk * 4 + 4);
    k_weights_w<<<(e->nk + 255) / 256, 256, 0, e->stream>>>(e->nk, kc, cs, e->d_wk.p);
    ewald_tables(e, w);
-   const int npart = e->ew_npart;
-   e->d_fpart.reserve((size_t)npart * 3 * e->nsp);
-   const EfArgs ef = force_args(e, e->nsites);
-   dim3 g((e->nsites + 255) / 256, npart);
-   k_ewald_force<<<g, 256, 0, e->stream>>>(ef);
-   k_add_parts<<<(e->nsites + 255) / 256, 256, 0, e->stream>>>(e->nsites, e->nsp, npart, e->d_fpart.p, out);
+   backproject_fp64(e, e->nsites, out, false);
 }
 
 void mgpu::ewald_charge_adjoint(mgpu_engine *e, const double *v, const double *dq_dev, double *dE, double *vjp,

@@ -166,6 +166,8 @@ struct mgpu_engine {
    mgpu::DBuf<double> d_wk;                     // per-k weights
    mgpu::DBuf<int> d_rpart;                     // row partitions of the force pass
    mgpu::DBuf<double> d_fpart;                  // per-partition site forces
+   mgpu::DBuf<double> d_zt64;                   // FP64 back-projection: site-major Z operand
+   mgpu::DBuf<double> d_w64;                    // FP64 back-projection: folded weight operands
    mgpu::DBuf<uint8_t> d_adj;                   // scratch of the parameter adjoint (k space)
    mgpu::DBuf<uint8_t> d_adj_real;              // scratch of the parameter adjoint (real space)
    int sk_chunks = 0;
