@@ -291,7 +291,7 @@ def main():
     res["autotuned_shapes"] = {"fp64": dict(fp64_backprojection_parts=tuned["fp64"][1])}
     if "mixed" in tuned:
         res["autotuned_shapes"]["mixed"] = dict(sk_workgroups=tuned["mixed"][0],
-                                                backprojection_parts=tuned["mixed"][1])
+                                                backprojection_rounds=tuned["mixed"][1])
     res["fp64_evals_per_s"] = 1.0 / t64
     res["fp64_ms_per_eval"] = 1e3 * t64
     if mixed:

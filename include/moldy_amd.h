@@ -223,9 +223,11 @@ int mgpu_adjoint(mgpu_engine *eng, const double *v, const double *dtheta, double
  * with the engines' own real/reciprocal stream overlap and with it off;
  * ms_per_step[3] receives the best wall time per step of each schedule.
  * mgpu_autotune picks e's launch shapes by timing its force evaluation:
- * the S(k) MFMA workgroup count and the MFMA back-projection row partitions
- * (mixed; returned in sk_blocks and npart), or the FP64 back-projection row
- * partitions (npart); results are identical up to the order of partial sums. */
+ * the S(k) MFMA workgroup count and the MFMA back-projection grid (mixed;
+ * returned in sk_blocks and npart: 0 = one workgroup per 128-site block,
+ * r > 0 = r rounds of resident workgroups sharing the work equally), or the
+ * FP64 back-projection row partitions (npart); results are identical up to
+ * the order of partial sums. */
 enum { MGPU_SCHED_GRAPH = 0, MGPU_SCHED_STREAM_GRAPHS = 1, MGPU_SCHED_STREAMS = 2 };
 typedef struct mgpu_batch mgpu_batch;
 int  mgpu_batch_create(mgpu_batch **batch, int device);

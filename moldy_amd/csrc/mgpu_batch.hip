@@ -212,7 +212,7 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       if (e->precision == MGPU_MIXED) {
          const int slots = e->ncu;   // one S(k) workgroup per CU (LDS)
          tune(e->tune_sk_blocks, {2 * slots, 3 * slots, 4 * slots, 6 * slots, 8 * slots, 12 * slots});
-         tune(e->tune_force_parts, {1, 2, 4, 8, 16});
+         tune(e->tune_force_parts, {0, 1, 2, 4});
       } else {
          tune(e->tune_npart, {4, 8, 16, 32});
       }

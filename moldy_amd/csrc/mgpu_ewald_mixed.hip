@@ -77,7 +77,8 @@ constexpr int ZT_MAXK2 = 192;   // 2 * 16 * NB for NB <= 6
 
 __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__restrict__ site,
                                                    const double *__restrict__ chg, float2 *__restrict__ X,
-                                                   float2 *__restrict__ Y, half_t *__restrict__ Zhr,
+                                                   float2 *__restrict__ Y, float2 *__restrict__ Y16,
+                                                   half_t *__restrict__ Zhr,
                                                    half_t *__restrict__ Zlr, half_t *__restrict__ Zhi,
                                                    half_t *__restrict__ Zli, half_t *__restrict__ ZTh,
                                                    half_t *__restrict__ ZTl) {
@@ -93,34 +94,51 @@ __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__re
    if (i >= a.ns) {
       for (int n = 0; n <= a.hmax; n++) X[(size_t)n * a.nsp + i] = make_float2(0, 0);
       for (int n = 0; n <= a.kmax; n++) Y[(size_t)n * a.nsp + i] = make_float2(0, 0);
+      Y16[i] = make_float2(0, 0);
       for (int n = 0; n <= a.lmax; n++) {
          size_t o = (size_t)n * a.nsp + i;
          Zhr[o] = Zlr[o] = Zhi[o] = Zli[o] = (half_t)0;
       }
    } else {
       double x = site[i], y = site[a.nsp + i], z = site[2 * a.nsp + i], q = chg[i] * a.sZ;
-      for (int ax = 0; ax < 3; ax++) {
-         int nmax = ax == 0 ? a.hmax : (ax == 1 ? a.kmax : a.lmax);
-         double kr = a.ks[3 * ax] * x + a.ks[3 * ax + 1] * y + a.ks[3 * ax + 2] * z;
-         double s1, c1;
+      // exp(i n k*.r) for n = 0 .. nmax by the complex recurrence
+      auto axis = [&](int ax, double &s1, double &c1) {
+         const double kr = a.ks[3 * ax] * x + a.ks[3 * ax + 1] * y + a.ks[3 * ax + 2] * z;
          sincos(kr, &s1, &c1);
-         double cn = 1.0, sn = 0.0;
-         for (int n = 0; n <= nmax; n++) {
-            size_t o = (size_t)n * a.nsp + i;
-            if (ax == 0) X[o] = make_float2((float)cn, (float)sn);
-            else if (ax == 1) Y[o] = make_float2((float)cn, (float)sn);
-            else {
-               half_t hr, lr, hi, li;
-               split2(q * cn, hr, lr);
-               split2(q * sn, hi, li);
-               Zhr[o] = hr; Zlr[o] = lr; Zhi[o] = hi; Zli[o] = li;
-               rh[n] = hr; rl[n] = lr;
-               rh[ncol + n] = hi; rl[ncol + n] = li;
-            }
-            double cc = cn * c1 - sn * s1;
-            sn = sn * c1 + cn * s1;
-            cn = cc;
-         }
+      };
+      double s1, c1, cn = 1.0, sn = 0.0;
+      axis(0, s1, c1);
+      for (int n = 0; n <= a.hmax; n++) {
+         X[(size_t)n * a.nsp + i] = make_float2((float)cn, (float)sn);
+         const double cc = cn * c1 - sn * s1;
+         sn = sn * c1 + cn * s1;
+         cn = cc;
+      }
+      // the y recurrence runs to n = 16 at least: exp(16 i b*.r) steps the
+      // back-projection from one k column to the next
+      axis(1, s1, c1);
+      cn = 1.0; sn = 0.0;
+      const int ny = a.kmax > 16 ? a.kmax : 16;
+      for (int n = 0; n <= ny; n++) {
+         if (n <= a.kmax) Y[(size_t)n * a.nsp + i] = make_float2((float)cn, (float)sn);
+         if (n == 16) Y16[i] = make_float2((float)cn, (float)sn);
+         const double cc = cn * c1 - sn * s1;
+         sn = sn * c1 + cn * s1;
+         cn = cc;
+      }
+      axis(2, s1, c1);
+      cn = 1.0; sn = 0.0;
+      for (int n = 0; n <= a.lmax; n++) {
+         const size_t o = (size_t)n * a.nsp + i;
+         half_t hr, lr, hi, li;
+         split2(q * cn, hr, lr);
+         split2(q * sn, hi, li);
+         Zhr[o] = hr; Zlr[o] = lr; Zhi[o] = hi; Zli[o] = li;
+         rh[n] = hr; rl[n] = lr;
+         rh[ncol + n] = hi; rl[ncol + n] = li;
+         const double cc = cn * c1 - sn * s1;
+         sn = sn * c1 + cn * s1;
+         cn = cc;
       }
    }
    __syncthreads();
@@ -459,9 +477,10 @@ __device__ __forceinline__ float pow2_scale(unsigned bits, float target) {
 
 // ------------------- weight operand tables for the back-projection
 struct WtArgs {
-   int nrows, nrows16, ncol, lmax;
-   const double *Wd;      // [nrows][+l | -l][ncol][2]
-   half_t *Wt;            // [4 kinds][2 hi/lo][nrows16][2 ncol]
+   int nbp, ncol, lmax;
+   const int *src;        // [nbp] k row of each back-projection row, -1 for padding
+   const double *Wd;      // [k rows][+l | -l][ncol][2]
+   half_t *Wt;            // [4 kinds][2 hi/lo][nbp][2 ncol], back-projection row order
    const unsigned *maxbits;
 };
 
@@ -473,12 +492,13 @@ struct WtArgs {
 __global__ void k_mix_wtab(WtArgs a) {
    const int K2 = 2 * a.ncol;
    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-   if (idx >= a.nrows16 * K2) return;
+   if (idx >= a.nbp * K2) return;
    const int r = idx / K2, kk = idx - r * K2;
    const int part = kk >= a.ncol, l = kk - part * a.ncol;
    double wpr = 0, wpi = 0, wmr = 0, wmi = 0;
-   if (r < a.nrows) {
-      const double *d = a.Wd + 2 * (size_t)r * K2;
+   const int rs = a.src[r];
+   if (rs >= 0) {
+      const double *d = a.Wd + 2 * (size_t)rs * K2;
       wpr = d[2 * l];
       wpi = d[2 * l + 1];
       wmr = d[2 * (a.ncol + l)];
@@ -492,27 +512,52 @@ __global__ void k_mix_wtab(WtArgs a) {
       const double s = kind < 2 ? sW : sQ;
       half_t hi, lo;
       split2(v[kind] * s, hi, lo);
-      a.Wt[((size_t)(kind * 2 + 0) * a.nrows16 + r) * K2 + kk] = hi;
-      a.Wt[((size_t)(kind * 2 + 1) * a.nrows16 + r) * K2 + kk] = lo;
+      a.Wt[((size_t)(kind * 2 + 0) * a.nbp + r) * K2 + kk] = hi;
+      a.Wt[((size_t)(kind * 2 + 1) * a.nbp + r) * K2 + kk] = lo;
    }
 }
 
 // -------------------------------------- back-projection: forces on sites
+//
+// Row order.  The back-projection walks the (h, k) rows in its own order:
+// the k columns (k = -kmax .. kmax, each with h = 0 .. its largest h) are
+// dealt to the 16 row lanes of a tile round robin, so lane c takes columns
+// -kmax + c, -kmax + c + 16, ... in turn, and tile t holds each lane's t-th
+// row.  Within a column the phase exp(i(h a* + k b*).r) advances by
+// exp(i a*.r) per row; from the start of one column to the next it advances
+// by exp(16 i b*.r).  A lane's phase is therefore two complex multiplies away
+// from its last one, with no table loads inside the row loop.  Rows that are
+// not in the k set ((0, k < 0), gaps, and the tail that pads every lane to
+// the same length) carry zero weight and zero k.
 struct FArgs {
-   int s0, s1, nrows, nrows16, lmax, kmax, nsp;
+   int s0, s1, nbp, nsp;
    float c2, sZ;
-   const int2 *rowhk;       // [nrows16] (h, k)
-   const float4 *rowk;      // [nrows16] (kx, ky, kzt, 0)
-   const float2 *X, *Y;
+   const int2 *bphk;        // [nbp] (h, k) in back-projection order; h < 0 pads a lane's tail
+   const float4 *bpk;       // [nbp] (kx, ky, kzt, 0), zero for padding rows
+   const float2 *X, *Y, *Y16;
    const half_t *ZTh, *ZTl; // site-major Z operand [nsp][2 NCOL]
    const half_t *Wt;
    const unsigned *maxbits;
    double *force;           // [3][nsp]
-   int nparts;              // row partitions (blockIdx.x % nparts)
-   int direct;              // add into force (one partition, nothing to wait for)
-   float *fpart;            // [nparts][3][nsp] otherwise
+   int nblk, nwg;           // site blocks of BSITES; workgroups (see k_mix_force)
+   int direct;              // one workgroup per whole block, nothing to wait for: add into force
+   float *fpart;            // [segments][3][nsp] otherwise
+   int hstep;               // X row of exp(i a*.r): 1, or 0 when hmax = 0 (no h steps)
 };
 
+// Work units are (site block, row tile) pairs, block-major; workgroup w
+// takes units [w U / G, (w+1) U / G) (U units, G workgroups), so every
+// workgroup carries the same number of tiles and a grid of whole rounds of
+// the resident slots has no partial last round.  A workgroup's share is
+// cut into segments, one per site block it touches; the segment of block b
+// starting in workgroup w is b's segment number w - first_wg(b NT).
+__host__ __device__ __forceinline__ int first_wg(long long u, long long U, int G) {
+   return (int)(((u + 1) * G - 1) / U);   // the workgroup whose share holds unit u
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+   return make_float2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
+}
 
 // Four waves of 32 sites each (two m tiles of 16), 128 sites a workgroup and
 // two workgroups per CU (NB <= 3), so one workgroup's epilogue runs beside
@@ -528,224 +573,207 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
    constexpr int CPR = K2 / 8;         // 16-byte chunks per weight row
    constexpr int NST = K2 / 16;        // chunks per thread and tile (8 kinds x 16 rows x CPR over 256 threads)
    __shared__ __attribute__((aligned(16))) half_t sW[2][8][16][RW];
-   __shared__ float2 sY16[BSITES];     // exp(16 i b*.r) per site of the block
-   // exp(i k.r) of each lane's current row at its 8 sites, lane-private
-   __shared__ float2 sAV[4][2][4][64];
+   // per site of the block: exp(i a*.r) (next row of a column) and
+   // exp(16 i b*.r) (start of the lane's next column), side by side
+   __shared__ float2 sMul[BSITES][2];
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
-   // row partition p of the site block: consecutive workgroups take
-   // consecutive partitions, so each XCD's L2 holds one slice of the weights
-   const int part = blockIdx.x % a.nparts;
-   const int base = a.s0 + (blockIdx.x / a.nparts) * BSITES;
-   const int ntile_all = a.nrows16 / 16;
-   const int rt0 = part * ntile_all / a.nparts, ntile = (part + 1) * ntile_all / a.nparts;
-   // weight staging: thread (row tid / 16, c0 = tid % 16) moves chunks
-   // u = c0 + 16 j of its row's 8 CPR chunks (kind x hi/lo = u / CPR)
-   int soff[NST], doff[NST];
-#pragma unroll
-   for (int j = 0; j < NST; j++) {
-      const int u = (tid & 15) + 16 * j, reg = u / CPR, c = u - reg * CPR, row = tid >> 4;
-      soff[j] = (reg * a.nrows16 + row) * K2 + 8 * c;
-      doff[j] = (reg * 16 + row) * RW + 8 * c;
-   }
-   half8 wst[NST];
-   auto load_w = [&](int rt) {
-      const half_t *src = a.Wt + (size_t)rt * 16 * K2;
-#pragma unroll
-      for (int j = 0; j < NST; j++) wst[j] = *(const half8 *)(src + soff[j]);
-   };
-   auto store_w = [&](int buf) {
-      half_t *dst = &sW[buf][0][0][0];
-#pragma unroll
-      for (int j = 0; j < NST; j++) *(half8 *)(dst + doff[j]) = wst[j];
-   };
-   load_w(rt0);
-   // Z operand [site][Zr | Zi over l = 0 .. NCOL-1], held for the whole
-   // kernel.  Sites past s1 read the last valid row: their products land
-   // only in their own output rows, which are never stored.
-   half8 zh[2][NB], zl[2][NB];
-#pragma unroll
-   for (int m = 0; m < 2; m++) {
-      const int i = base + 32 * wave + 16 * m + lc, ic = i < a.s1 ? i : a.s1 - 1;
-      const size_t o = (size_t)ic * K2 + 8 * g;
-#pragma unroll
-      for (int ks = 0; ks < NB; ks++) {
-         zh[m][ks] = *(const half8 *)(a.ZTh + o + ks * 32);
-         zl[m][ks] = *(const half8 *)(a.ZTl + o + ks * 32);
-      }
-   }
+   const int ntile_all = a.nbp / 16;
+   const long long U = (long long)a.nblk * ntile_all;
+   const int wg = blockIdx.x;
+   long long u = (long long)wg * U / a.nwg;
+   const long long uend = (long long)(wg + 1) * U / a.nwg;
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
    const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
    const float c2q = a.c2 * (inv_qw / inv_pw);
-   float f[2][4][3];
+   while (u < uend) {
+      const int blk = (int)(u / ntile_all);
+      const int rt0 = (int)(u - (long long)blk * ntile_all);
+      const int ntile = (int)std::min<long long>(ntile_all, rt0 + (uend - u));
+      const int seg = wg - first_wg((long long)blk * ntile_all, U, a.nwg);
+      const int base = a.s0 + blk * BSITES;
+      u += ntile - rt0;
+      __syncthreads();   // the previous segment's LDS reads are done
+      // weight staging: thread (row tid / 16, c0 = tid % 16) moves chunks
+      // q = c0 + 16 j of its row's 8 CPR chunks (kind x hi/lo = q / CPR)
+      int soff[NST], doff[NST];
 #pragma unroll
-   for (int m = 0; m < 2; m++)
+      for (int j = 0; j < NST; j++) {
+         const int q = (tid & 15) + 16 * j, reg = q / CPR, c = q - reg * CPR, row = tid >> 4;
+         soff[j] = (reg * a.nbp + row) * K2 + 8 * c;
+         doff[j] = (reg * 16 + row) * RW + 8 * c;
+      }
+      half8 wst[NST];
+      auto load_w = [&](int rt) {
+         const half_t *src = a.Wt + (size_t)rt * 16 * K2;
 #pragma unroll
-      for (int j = 0; j < 4; j++) f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
-   if (tid < BSITES) {
-      const int i = base + tid;
-      sY16[tid] = (a.kmax >= 16 && i < a.s1) ? a.Y[(size_t)16 * a.nsp + i] : make_float2(0, 0);
-   }
-   int cur_h = -1, cur_k = -1000000;
-   // row metadata of the lane's row, one tile ahead of its use
-   int2 hk = a.rowhk[rt0 * 16 + lc];
-   float4 kv = a.rowk[rt0 * 16 + lc];
-   store_w(0);
-   load_w(rt0 + 1 < ntile ? rt0 + 1 : rt0);
-   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-   for (int rt = rt0; rt < ntile; rt++) {
-      const int buf = (rt - rt0) & 1;
-      const half_t *W = &sW[buf][0][0][0];
-      const int r = rt * 16 + lc;
-      // a lane whose row starts a new (h, k) run takes exp(i h a*.r) and
-      // exp(i |k| b*.r) of its 8 sites from the tables; the loads go out
-      // before the MFMAs and land under them (sites past s1 read the last
-      // valid site; their results are never stored)
-      const bool step16 = hk.x == cur_h && hk.y == cur_k + 16 && a.kmax >= 16;
-      float2 xs[2][4], ys[2][4];
-      if (r < a.nrows && !step16) {
-         const float2 *xrow = a.X + (size_t)hk.x * a.nsp, *yrow = a.Y + (size_t)(hk.y < 0 ? -hk.y : hk.y) * a.nsp;
+         for (int j = 0; j < NST; j++) wst[j] = *(const half8 *)(src + soff[j]);
+      };
+      auto store_w = [&](int buf) {
+         half_t *dst = &sW[buf][0][0][0];
+#pragma unroll
+         for (int j = 0; j < NST; j++) *(half8 *)(dst + doff[j]) = wst[j];
+      };
+      load_w(rt0);
+      // Z operand [site][Zr | Zi over l = 0 .. NCOL-1], held for the whole
+      // segment.  Sites past s1 read the last valid row: their products land
+      // only in their own output rows, which are never stored.
+      half8 zh[2][NB], zl[2][NB];
+#pragma unroll
+      for (int m = 0; m < 2; m++) {
+         const int i = base + 32 * wave + 16 * m + lc, ic = i < a.s1 ? i : a.s1 - 1;
+         const size_t o = (size_t)ic * K2 + 8 * g;
+#pragma unroll
+         for (int ks = 0; ks < NB; ks++) {
+            zh[m][ks] = *(const half8 *)(a.ZTh + o + ks * 32);
+            zl[m][ks] = *(const half8 *)(a.ZTl + o + ks * 32);
+         }
+      }
+      float f[2][4][3];
+#pragma unroll
+      for (int m = 0; m < 2; m++)
+#pragma unroll
+         for (int j = 0; j < 4; j++) f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
+      if (tid < BSITES) {
+         const int i = base + tid, ic = i < a.s1 ? i : a.s1 - 1;
+         sMul[tid][0] = a.X[(size_t)a.hstep * a.nsp + ic];
+         sMul[tid][1] = a.Y16[ic];
+      }
+      // the lane's first row: phase A = exp(i(h a* + k b*).r) and the phase of
+      // its column's start B = exp(i k b*.r) (k < 0 through the conjugate of
+      // exp(i |k| b*.r)) from the tables, for its 8 sites (sites past s1 read
+      // the last valid site; their results are never stored)
+      float2 A[2][4], B[2][4];
+      {
+         int2 hk0 = a.bphk[rt0 * 16 + lc];
+         if (hk0.x < 0) hk0 = make_int2(0, 0);
+         const float2 *xrow = a.X + (size_t)hk0.x * a.nsp, *yrow = a.Y + (size_t)(hk0.y < 0 ? -hk0.y : hk0.y) * a.nsp;
+         const float ysg = hk0.y < 0 ? -1.f : 1.f;
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                const int i = base + 32 * wave + 16 * m + 4 * g + j, ic = i < a.s1 ? i : a.s1 - 1;
-               xs[m][j] = xrow[ic];
-               ys[m][j] = yrow[ic];
+               const float2 y = yrow[ic];
+               B[m][j] = make_float2(y.x, ysg * y.y);
+               A[m][j] = cmulf(xrow[ic], B[m][j]);
             }
       }
-      floatx4 acc[2][4];
+      float4 kv = a.bpk[rt0 * 16 + lc];
+      store_w(0);
+      load_w(rt0 + 1 < ntile ? rt0 + 1 : rt0);
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      for (int rt = rt0; rt < ntile; rt++) {
+         const int buf = (rt - rt0) & 1;
+         const half_t *W = &sW[buf][0][0][0];
+         floatx4 acc[2][4];
 #pragma unroll
-      for (int m = 0; m < 2; m++)
+         for (int m = 0; m < 2; m++)
 #pragma unroll
-         for (int kd = 0; kd < 4; kd++) acc[m][kd] = floatx4{0, 0, 0, 0};
+            for (int kd = 0; kd < 4; kd++) acc[m][kd] = floatx4{0, 0, 0, 0};
 #pragma unroll
-      for (int p = 0; p < 4 * NB; p++) {
-         const int ks = p >> 2, kd = p & 3;
-         const half8 bh = *(const half8 *)(W + ((2 * kd) * 16 + lc) * RW + ks * 32 + 8 * g);
-         const half8 bl = *(const half8 *)(W + ((2 * kd + 1) * 16 + lc) * RW + ks * 32 + 8 * g);
+         for (int p = 0; p < 4 * NB; p++) {
+            const int ks = p >> 2, kd = p & 3;
+            const half8 bh = *(const half8 *)(W + ((2 * kd) * 16 + lc) * RW + ks * 32 + 8 * g);
+            const half8 bl = *(const half8 *)(W + ((2 * kd + 1) * 16 + lc) * RW + ks * 32 + 8 * g);
 #pragma unroll
-         for (int m = 0; m < 2; m++) {
-            acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
-            acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
-            acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
+            for (int m = 0; m < 2; m++) {
+               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
+               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
+               acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
+            }
          }
-      }
-      // issue order: the weight fragments of pair p + 1 (two LDS reads) go
-      // out before the six MFMAs of pair p, hiding the LDS latency
-      __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
+         // issue order: the weight fragments of pair p + 1 (two LDS reads) go
+         // out before the six MFMAs of pair p, hiding the LDS latency
+         __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
 #pragma unroll
-      for (int p = 0; p < 4 * NB; p++) {
-         __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
-         if (p + 2 < 4 * NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
-      }
-      // phase state and exp(16 i b*.r) of the lane's 8 sites
-      float2 yv[2][4], sv[2][4];
-#pragma unroll
-      for (int m = 0; m < 2; m++)
-#pragma unroll
-         for (int j = 0; j < 4; j++) {
-            yv[m][j] = sAV[wave][m][j][lane];
-            sv[m][j] = sY16[32 * wave + 16 * m + 4 * g + j];
+         for (int p = 0; p < 4 * NB; p++) {
+            __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
+            if (p + 2 < 4 * NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
          }
-      const int rnext = rt + 1 < ntile ? rt + 1 : rt;
-      const int2 hk_n = a.rowhk[rnext * 16 + lc];
-      const float4 kv_n = a.rowk[rnext * 16 + lc];
-      if (r < a.nrows) {
-         if (step16) {
-            // exp(i (k+16) b*.r) = exp(i k b*.r) exp(i 16 b*.r)
-#pragma unroll
-            for (int m = 0; m < 2; m++)
-#pragma unroll
-               for (int j = 0; j < 4; j++) {
-                  const float2 y = yv[m][j], s = sv[m][j];
-                  yv[m][j] = make_float2(y.x * s.x - y.y * s.y, y.y * s.x + y.x * s.y);
-               }
-         } else {
-            // a new (h, k) run: exp(i h a*.r) exp(i k b*.r), k < 0 through
-            // the conjugate of exp(i |k| b*.r)
-            const float ysg = hk.y < 0 ? -1.f : 1.f;
-#pragma unroll
-            for (int m = 0; m < 2; m++)
-#pragma unroll
-               for (int j = 0; j < 4; j++) {
-                  const float2 x = xs[m][j], y = ys[m][j];
-                  const float yi = ysg * y.y;
-                  yv[m][j] = make_float2(x.x * y.x - x.y * yi, x.y * y.x + x.x * yi);
-               }
-         }
+         const int rnext = rt + 1 < ntile ? rt + 1 : rt;
+         const int2 hk_n = a.bphk[rnext * 16 + lc];
+         const float4 kv_n = a.bpk[rnext * 16 + lc];
          // P and Q carry power-of-two scales: the sums run in units of the
          // P scale (exact, so the same bits as scaling every term) and the Q
-         // terms join through c2 times the ratio of the two scales
+         // terms join through c2 times the ratio of the two scales.  Padding
+         // rows have zero weights and zero k, so they add exact zeros.
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-               const float Ar = yv[m][j].x, Ai = yv[m][j].y;
+               const float Ar = A[m][j].x, Ai = A[m][j].y;
                const float t1 = Ar * acc[m][1][j] + Ai * acc[m][0][j];
                const float t2 = Ar * acc[m][3][j] + Ai * acc[m][2][j];
                f[m][j][0] += kv.x * t1;
                f[m][j][1] += kv.y * t1;
                f[m][j][2] += kv.z * t1 + c2q * t2;
             }
-         cur_h = hk.x;
-         cur_k = hk.y;
+         // the phase of the lane's next row: h + 1 in the same column, or the
+         // start (h = 0) of the column k + 16
+         const bool cs = hk_n.x == 0;
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) sAV[wave][m][j][lane] = yv[m][j];
+            for (int j = 0; j < 4; j++) {
+               const float2 mu = sMul[32 * wave + 16 * m + 4 * g + j][cs];
+               const float2 na = cmulf(cs ? B[m][j] : A[m][j], mu);
+               A[m][j] = na;
+               B[m][j] = cs ? na : B[m][j];
+            }
+         // tile rt + 1 into the other buffer (last read in tile rt - 1, before
+         // the previous barrier), tile rt + 2 into the registers
+         store_w(buf ^ 1);
+         load_w(rt + 2 < ntile ? rt + 2 : ntile - 1);
+         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+         kv = kv_n;
       }
-      // tile rt + 1 into the other buffer (last read in tile rt - 1, before
-      // the previous barrier), tile rt + 2 into the registers
-      store_w(buf ^ 1);
-      load_w(rt + 2 < ntile ? rt + 2 : ntile - 1);
-      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-      hk = hk_n;
-      kv = kv_n;
-   }
-   // lanes with the same g hold the same sites: sum over the 16 row lanes
-#pragma unroll
-   for (int m = 0; m < 2; m++)
-#pragma unroll
-      for (int j = 0; j < 4; j++)
-#pragma unroll
-         for (int c = 0; c < 3; c++) {
-            float v = f[m][j][c];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            v += __shfl_xor(v, 8, 64);
-            f[m][j][c] = v * inv_pw;
-         }
-   if (lc == 0) {
+      // lanes with the same g hold the same sites: sum over the 16 row lanes
 #pragma unroll
       for (int m = 0; m < 2; m++)
 #pragma unroll
-         for (int j = 0; j < 4; j++) {
-            const int i = base + 32 * wave + 16 * m + 4 * g + j;
-            if (i < a.s1) {
-               if (a.direct) {
-                  a.force[i] += (double)f[m][j][0];
-                  a.force[a.nsp + i] += (double)f[m][j][1];
-                  a.force[2 * a.nsp + i] += (double)f[m][j][2];
-               } else {
-                  float *fp = a.fpart + (size_t)part * 3 * a.nsp;
-                  fp[i] = f[m][j][0];
-                  fp[a.nsp + i] = f[m][j][1];
-                  fp[2 * a.nsp + i] = f[m][j][2];
+         for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+               float v = f[m][j][c];
+               v += __shfl_xor(v, 1, 64);
+               v += __shfl_xor(v, 2, 64);
+               v += __shfl_xor(v, 4, 64);
+               v += __shfl_xor(v, 8, 64);
+               f[m][j][c] = v * inv_pw;
+            }
+      if (lc == 0) {
+#pragma unroll
+         for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+               const int i = base + 32 * wave + 16 * m + 4 * g + j;
+               if (i < a.s1) {
+                  if (a.direct) {
+                     a.force[i] += (double)f[m][j][0];
+                     a.force[a.nsp + i] += (double)f[m][j][1];
+                     a.force[2 * a.nsp + i] += (double)f[m][j][2];
+                  } else {
+                     float *fp = a.fpart + (size_t)seg * 3 * a.nsp;
+                     fp[i] = f[m][j][0];
+                     fp[a.nsp + i] = f[m][j][1];
+                     fp[2 * a.nsp + i] = f[m][j][2];
+                  }
                }
             }
-         }
+      }
    }
 }
 
-// Partition partials into the site forces, in partition order.
-__global__ void k_mix_fsum(int s0, int s1, int nsp, int nparts, const float *__restrict__ fpart,
-                           double *__restrict__ force) {
+// Segment partials into the site forces, in segment order.
+__global__ void k_mix_fsum(int s0, int s1, int nsp, int ntile_all, long long U, int nwg,
+                           const float *__restrict__ fpart, double *__restrict__ force) {
    const int i = s0 + blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= s1) return;
+   const long long b0 = (long long)((i - s0) / BSITES) * ntile_all;
+   const int nseg = first_wg(b0 + ntile_all - 1, U, nwg) - first_wg(b0, U, nwg) + 1;
    for (int c = 0; c < 3; c++) {
       double t = 0.0;
-      for (int p = 0; p < nparts; p++) t += (double)fpart[((size_t)p * 3 + c) * nsp + i];
+      for (int p = 0; p < nseg; p++) t += (double)fpart[((size_t)p * 3 + c) * nsp + i];
       force[(size_t)c * nsp + i] += t;
    }
 }
@@ -771,6 +799,46 @@ template <int NB> void launch_force(dim3 g, size_t, hipStream_t s, const FArgs &
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Back-projection row order: column k (k = -kmax .. kmax, index j = k + kmax)
+// holds h = 0 .. the largest h of the k set's rows with that k, at least
+// one row; columns go to row lanes j mod 16 in increasing k, and every lane
+// is padded to the longest lane's length.  Row 16 t + c is lane c's t-th row.
+void plan_backprojection(mgpu_engine *e) {
+   if (e->bp_version == e->geom_version && !e->bp_src.empty()) return;
+   const int kmax = e->kmax, ncols = 2 * kmax + 1, hmax = e->hmax;
+   std::vector<int> top(ncols, 0), at((size_t)(hmax + 1) * ncols, -1);
+   for (int r = 0; r < e->nrows; r++) {
+      const KRow &R = e->rows[r];
+      const int j = R.k + kmax;
+      top[j] = std::max(top[j], R.h);
+      at[(size_t)R.h * ncols + j] = r;
+   }
+   int len = 0;
+   for (int c = 0; c < 16; c++) {
+      int n = 0;
+      for (int j = c; j < ncols; j += 16) n += top[j] + 1;
+      len = std::max(len, n);
+   }
+   const int nbp = 16 * len;
+   e->bp_src.assign(nbp, -1);
+   e->bp_hk.assign(nbp, make_int2(-1, 0));
+   e->bp_k.assign(nbp, make_float4(0, 0, 0, 0));
+   for (int c = 0; c < 16; c++) {
+      int t = 0;
+      for (int j = c; j < ncols; j += 16)
+         for (int h = 0; h <= top[j]; h++, t++) {
+            const int rb = 16 * t + c, r = at[(size_t)h * ncols + j];
+            e->bp_hk[rb] = make_int2(h, j - kmax);
+            e->bp_src[rb] = r;
+            if (r >= 0) {
+               const KRow &R = e->rows[r];
+               e->bp_k[rb] = make_float4((float)R.kx, (float)R.ky, (float)R.kzt, 0.f);
+            }
+         }
+   }
+   e->bp_version = e->geom_version;
+}
+
 }  // namespace
 
 void mgpu::run_ewald_mixed(mgpu_engine *e) {
@@ -780,7 +848,9 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    const int NCOL = 16 * NB, K2 = 2 * NCOL;
    const int nrows = e->nrows;
    const int ntile = (nrows + FROWS - 1) / FROWS;
-   const int nrows64 = ntile * FROWS, nrows16 = (nrows + 15) / 16 * 16;
+   const int nrows64 = ntile * FROWS;
+   plan_backprojection(e);
+   const int nbp = (int)e->bp_src.size();
    // site slices: 32-aligned starts, never mixing framework and other sites
    std::vector<int4> slices;
    std::vector<int> slice_fw;
@@ -804,25 +874,45 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return o; };
    const size_t oX = take(sizeof(float2) * (e->hmax + 1) * (size_t)nsp);
    const size_t oY = take(sizeof(float2) * (e->kmax + 1) * (size_t)nsp);
+   const size_t oY16 = take(sizeof(float2) * (size_t)nsp);
    const size_t oZ = take(sizeof(half_t) * 4 * (lmax + 1) * (size_t)nsp);
    const size_t oZT = take(sizeof(half_t) * 2 * (size_t)K2 * nsp);
    const size_t oP = take(sizeof(float) * (size_t)nslices * ntile * 4 * FROWS * NCOL);
    const size_t oWd = take(sizeof(double) * 2 * (size_t)nrows * K2);
    const size_t oWn = take(sizeof(double) * 2 * (size_t)nrows * K2);
-   const size_t oWt = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
-   const size_t oWtn = take(sizeof(half_t) * 8 * (size_t)nrows16 * K2);
+   const size_t oWt = take(sizeof(half_t) * 8 * (size_t)nbp * K2);
+   const size_t oWtn = take(sizeof(half_t) * 8 * (size_t)nbp * K2);
    const size_t oHK = take(sizeof(int2) * nrows64);
-   const size_t oRK = take(sizeof(float4) * nrows64);
+   const size_t oBS = take(sizeof(int) * nbp);
+   const size_t oBHK = take(sizeof(int2) * nbp);
+   const size_t oBK = take(sizeof(float4) * nbp);
    const size_t oSl = take(sizeof(int4) * nslices);
    const size_t oSf = take(sizeof(int) * nslices);
    const size_t oMx = take(sizeof(unsigned) * 4);
    const int nred = (nrows * K2 + 255) / 256;
    const size_t oBp = take(sizeof(double) * 8 * (size_t)nred);
-   const int nparts = std::max(1, std::min(e->tune_force_parts, nrows16 / 16));
-   const size_t oFp = take(sizeof(float) * 3 * (size_t)nparts * nsp);
+   // back-projection grid: tune_force_parts = 0 gives one workgroup per site
+   // block (whole blocks, added straight into the forces when nothing is
+   // pending); r > 0 gives r rounds of the resident slots, at least 8 row
+   // tiles a workgroup, each workgroup an equal share of the units
+   struct FGrid { int nblk, nwg, maxseg; };
+   const int ntile_bp = nbp / 16;
+   const int fslots = (NB <= 3 ? 2 : 1) * e->ncu;
+   auto force_grid = [&](int s0, int s1) {
+      FGrid fg{(s1 - s0 + BSITES - 1) / BSITES, 0, 1};
+      if (fg.nblk <= 0) return fg;
+      const long long U = (long long)fg.nblk * ntile_bp;
+      fg.nwg = e->tune_force_parts <= 0
+                   ? fg.nblk
+                   : (int)std::max<long long>(1, std::min<long long>((long long)e->tune_force_parts * fslots, U / 8));
+      fg.maxseg = (fg.nwg + fg.nblk - 1) / fg.nblk + 1;
+      return fg;
+   };
+   const FGrid fgn = force_grid(0, e->nsitesxf), fgf = force_grid(e->nsitesxf, ns);
+   const size_t oFp = take(sizeof(float) * 3 * (size_t)std::max(fgn.maxseg, fgf.maxseg) * nsp);
    e->d_mix.reserve(off);
    uint8_t *B = e->d_mix.p;
-   float2 *X = (float2 *)(B + oX), *Y = (float2 *)(B + oY);
+   float2 *X = (float2 *)(B + oX), *Y = (float2 *)(B + oY), *Y16 = (float2 *)(B + oY16);
    half_t *Zhr = (half_t *)(B + oZ), *Zlr = Zhr + (size_t)(lmax + 1) * nsp,
           *Zhi = Zlr + (size_t)(lmax + 1) * nsp, *Zli = Zhi + (size_t)(lmax + 1) * nsp;
    half_t *ZTh = (half_t *)(B + oZT), *ZTl = ZTh + (size_t)K2 * nsp;
@@ -830,7 +920,9 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    double *Wd = (double *)(B + oWd), *Wnd = (double *)(B + oWn);
    half_t *Wt = (half_t *)(B + oWt), *Wtn = (half_t *)(B + oWtn);
    int2 *rowhk = (int2 *)(B + oHK);
-   float4 *rowk = (float4 *)(B + oRK);
+   int *bpsrc = (int *)(B + oBS);
+   int2 *bphk = (int2 *)(B + oBHK);
+   float4 *bpk = (float4 *)(B + oBK);
    int4 *dsl = (int4 *)(B + oSl);
    int *dsf = (int *)(B + oSf);
    unsigned *maxbits = (unsigned *)(B + oMx);
@@ -840,13 +932,11 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    // or the scratch block changes
    if (e->mix_plan_version != e->geom_version || e->mix_plan_ptr != (void *)B) {
       std::vector<int2> hk(nrows64, make_int2(-1, 0));
-      std::vector<float4> rk(nrows64, make_float4(0, 0, 0, 0));
-      for (int r = 0; r < nrows; r++) {
-         hk[r] = make_int2(e->rows[r].h, e->rows[r].k);
-         rk[r] = make_float4((float)e->rows[r].kx, (float)e->rows[r].ky, (float)e->rows[r].kzt, 0.f);
-      }
+      for (int r = 0; r < nrows; r++) hk[r] = make_int2(e->rows[r].h, e->rows[r].k);
       HIPCHK(hipMemcpyAsync(rowhk, hk.data(), sizeof(int2) * nrows64, hipMemcpyHostToDevice, e->stream));
-      HIPCHK(hipMemcpyAsync(rowk, rk.data(), sizeof(float4) * nrows64, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(bpsrc, e->bp_src.data(), sizeof(int) * nbp, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(bphk, e->bp_hk.data(), sizeof(int2) * nbp, hipMemcpyHostToDevice, e->stream));
+      HIPCHK(hipMemcpyAsync(bpk, e->bp_k.data(), sizeof(float4) * nbp, hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipMemcpyAsync(dsl, slices.data(), sizeof(int4) * nslices, hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipMemcpyAsync(dsf, slice_fw.data(), sizeof(int) * nslices, hipMemcpyHostToDevice, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
@@ -862,8 +952,8 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       TabArgs t;
       for (int i = 0; i < 9; i++) t.ks[i] = 2 * PI_D * e->hinv[i];
       t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = lmax; t.ns = ns; t.nsp = nsp; t.ncol = NCOL; t.sZ = sZ;
-      k_mix_tables<<<nsp / ZT_SITES, ZT_SITES, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Zhr, Zlr,
-                                                               Zhi, Zli, ZTh, ZTl);
+      k_mix_tables<<<nsp / ZT_SITES, ZT_SITES, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Y16, Zhr,
+                                                               Zlr, Zhi, Zli, ZTh, ZTl);
    }
    {
       ProfScope ps(e, "ewald_mfma_sk");
@@ -885,42 +975,43 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       r.rows = e->d_rows.p; r.slice_fw = dsf; r.part = part; r.Wd = Wd; r.Wnd = Wnd; r.w = e->d_wk.p;
       r.bpart = bpart; r.maxbits = maxbits;
       k_mix_reduce<<<nred, 256, 0, e->stream>>>(r);
-      WtArgs w{nrows, nrows16, NCOL, lmax, Wd, Wt, maxbits};
-      int nw = nrows16 * K2;
+      WtArgs w{nbp, NCOL, lmax, bpsrc, Wd, Wt, maxbits};
+      int nw = nbp * K2;
       k_mix_wtab<<<(nw + 255) / 256, 256, 0, e->stream>>>(w);
       if (e->nsitesxf < ns) {
-         WtArgs wn{nrows, nrows16, NCOL, lmax, Wnd, Wtn, maxbits};
+         WtArgs wn{nbp, NCOL, lmax, bpsrc, Wnd, Wtn, maxbits};
          k_mix_wtab<<<(nw + 255) / 256, 256, 0, e->stream>>>(wn);
       }
    }
    {
       ProfScope ps(e, "ewald_mfma_force");
       FArgs f;
-      f.nrows = nrows; f.nrows16 = nrows16; f.lmax = lmax; f.kmax = e->kmax; f.nsp = nsp;
+      f.nbp = nbp; f.nsp = nsp;
       f.c2 = (float)(2 * PI_D * e->hinv[8]);
       f.sZ = (float)sZ;
-      f.rowhk = rowhk; f.rowk = rowk; f.X = X; f.Y = Y;
+      f.bphk = bphk; f.bpk = bpk; f.X = X; f.Y = Y; f.Y16 = Y16;
+      f.hstep = e->hmax >= 1 ? 1 : 0;
       f.ZTh = ZTh; f.ZTl = ZTl; f.maxbits = maxbits;
       f.force = e->d_force_site.p;
-      f.nparts = nparts;
-      f.direct = nparts == 1 && !e->join_pending;
       f.fpart = fpart;
       const size_t shm = 0;
-      auto run = [&](int s0, int s1, const half_t *wt) {
+      auto run = [&](int s0, int s1, const half_t *wt, const FGrid &fg) {
          if (s1 <= s0) return;
          f.s0 = s0; f.s1 = s1; f.Wt = wt;
-         dim3 g((unsigned)(((s1 - s0 + BSITES - 1) / BSITES) * nparts));
+         f.nblk = fg.nblk; f.nwg = fg.nwg;
+         f.direct = fg.nwg == fg.nblk && !e->join_pending;
+         dim3 g((unsigned)fg.nwg);
 #define CALL_F(nb) launch_force<nb>(g, shm, e->stream, f)
          NB_SWITCH(NB, CALL_F)
 #undef CALL_F
          if (!f.direct) {
             join_side(e);
-            k_mix_fsum<<<(s1 - s0 + 255) / 256, 256, 0, e->stream>>>(s0, s1, nsp, nparts, fpart,
-                                                                     e->d_force_site.p);
+            k_mix_fsum<<<(s1 - s0 + 255) / 256, 256, 0, e->stream>>>(
+                s0, s1, nsp, ntile_bp, (long long)fg.nblk * ntile_bp, fg.nwg, fpart, e->d_force_site.p);
          }
       };
-      run(0, e->nsitesxf, Wt);
-      run(e->nsitesxf, ns, Wtn);
+      run(0, e->nsitesxf, Wt, fgn);
+      run(e->nsitesxf, ns, Wtn, fgf);
    }
    e->d_out.reserve(OUT_SIZE);
    reduce_cols(e, bpart, nred, 8, e->d_out.p + OUT_RECIP);

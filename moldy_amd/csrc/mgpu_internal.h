@@ -175,6 +175,12 @@ struct mgpu_engine {
    // ---- mixed-precision ewald ----
    mgpu::DBuf<uint8_t> d_mix;                   // scratch for the MFMA path
    size_t mix_bytes = 0;
+   // MFMA back-projection row order (k columns dealt to 16 row lanes), built
+   // on the host from the k set when the geometry version changes
+   std::vector<int> bp_src;                     // k row of each back-projection row, -1 padding
+   std::vector<int2> bp_hk;                     // (h, k); h < 0 pads a lane's tail
+   std::vector<float4> bp_k;                    // (kx, ky, kzt, 0), zero for padding
+   int bp_version = -1;
 
    // ---- plan caching: uploads of k-set/slice metadata happen when the
    // geometry version changes, so a steady-state evaluation is launches only
@@ -185,7 +191,7 @@ struct mgpu_engine {
    int ncu = 256;                               // compute units of the device
    int tune_sk_blocks = 1024;                   // most S(k) MFMA workgroups (2 per CU resident)
    int tune_npart = 16;                         // FP64 back-projection row partitions
-   int tune_force_parts = 8;                    // MFMA back-projection row partitions
+   int tune_force_parts = 1;                    // MFMA back-projection: rounds of resident workgroups (0: one per site block)
    void *mix_plan_ptr = nullptr;
    bool capturing = false;                      // inside hipStreamBeginCapture
 
