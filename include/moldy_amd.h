@@ -84,13 +84,16 @@ int  mgpu_set_overlap(mgpu_engine *eng, int on);
 /* Real-space kernel: 0 automatic (the load-balanced kernel when the grid
  * holds at least 16 of its waves per CU), 1 the per-offset walk, 2 the
  * load-balanced kernel (one wave per site over the shell's z-columns of
- * cells) wherever the shell splits into columns, else the walk.  Both give
- * the reference's pair set; only the summation order differs.
+ * cells) wherever the shell splits into columns, else the walk; 3 as 2 with
+ * every site's candidates assigned to their column runs by binary search,
+ * the path taken by sites with more than 2048 candidates.  All give the
+ * reference's pair set; only the summation order differs.
  * mgpu_realspace_balanced reports whether the last real-space pass used the
  * load-balanced kernel.  No counterpart in the reference. */
 #define MGPU_RS_AUTO 0
 #define MGPU_RS_WALK 1
 #define MGPU_RS_BALANCED 2
+#define MGPU_RS_BALANCED_SEARCH 3
 int  mgpu_set_realspace_mode(mgpu_engine *eng, int mode);
 int  mgpu_realspace_balanced(const mgpu_engine *eng);
 

@@ -757,7 +757,7 @@ int mgpu_set_overlap(mgpu_engine *e, int on) {
 }
 
 int mgpu_set_realspace_mode(mgpu_engine *e, int mode) {
-   if (!e || mode < MGPU_RS_AUTO || mode > MGPU_RS_BALANCED) return MGPU_E_ARG;
+   if (!e || mode < MGPU_RS_AUTO || mode > MGPU_RS_BALANCED_SEARCH) return MGPU_E_ARG;
    if (e->rs_mode != mode) {
       e->rs_mode = mode;
       e->geom_version++;   // captured step graphs follow the kernel choice

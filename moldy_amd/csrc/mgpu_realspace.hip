@@ -41,6 +41,7 @@ struct RSArgs {
    int nsites;
    // z-columns of the shell for the load-balanced pass
    int ncol, own_col;
+   int bits_cap;          // most candidates of a site placed by piece-start bits (else binary search)
    const int4 *cols;      // dx, dy, dz lo, dz hi
    const float4 *colD;    // h (dx/nx, dy/ny, 0)
    const int *ccz;        // z index of each cell-ordered site's cell
@@ -219,20 +220,35 @@ __global__ __launch_bounds__(256) void k_realspace(RSArgs a) {
 // cell occupancies, where the per-offset walk above waits on the fullest
 // cell of the wave at every offset.  The pair set and the pair term are the
 // walk's; only the summation order differs.
-constexpr int RSB_MAXP = 512;   // pieces per site: two per column, at most 256 columns
+constexpr int RSB_NBW = 32;     // 64-bit words of piece-start bits per wave (sites of up to 2048 candidates)
+
+// Dynamic LDS of the balanced kernel, for maxp = 2 ncol pieces per site and
+// npp potential parameters: reloc (FP64), per wave the piece table, the
+// piece-start bits and the piece starts, then the potential table.
+__host__ __device__ __forceinline__ size_t rsb_lds_bytes(int maxp, int npp, int rsize) {
+   return 81 * 8 + 4 * (size_t)maxp * 8 + 4 * RSB_NBW * 8 + 4 * (size_t)(maxp + 1) * 4 + (size_t)npp * rsize;
+}
 
 template <class R, int PT, bool COUL>
 __global__ __launch_bounds__(256) void k_realspace_bal(RSArgs a) {
-   __shared__ R spot[MGPU_MAX_ID * MGPU_MAX_ID * MGPU_NPOTP];
-   __shared__ double sreloc[81];
-   __shared__ int sst[4][RSB_MAXP + 1];   // first candidate of each piece, then the total
-   __shared__ int2 spd[4][RSB_MAXP];      // (site index - first candidate, column | z field << 10)
-   const int npp = a.max_id * a.max_id * MGPU_NPOTP;
+   extern __shared__ __attribute__((aligned(16))) unsigned char rsb_lds[];
+   const int npp = a.max_id * a.max_id * MGPU_NPOTP, maxp = 2 * a.ncol;
+   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   // carved in the order of rsb_lds_bytes
+   unsigned char *lp = rsb_lds;
+   double *sreloc = (double *)lp;
+   lp += 81 * 8;
+   int2 *spd = (int2 *)lp + wave * maxp;   // (site index - first candidate, column | z field << 10)
+   lp += 4 * (size_t)maxp * 8;
+   unsigned long long *sbits = (unsigned long long *)lp + wave * RSB_NBW;   // piece-start bits
+   lp += 4 * RSB_NBW * 8;
+   int *sst = (int *)lp + wave * (maxp + 1);   // first candidate of each piece, then the total
+   lp += 4 * (size_t)(maxp + 1) * 4;
+   R *spot = (R *)lp;
    for (int t = threadIdx.x; t < npp; t += blockDim.x) spot[t] = (R)a.potpar[t];
    for (int t = threadIdx.x; t < 81; t += blockDim.x) sreloc[t] = a.reloc[t];
    __syncthreads();
 
-   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    const int p = blockIdx.x * 4 + wave;
    double *pp = a.part + 8 * (size_t)p;
    if (p >= a.nsites) {   // wave-uniform; no block barrier follows
@@ -300,17 +316,26 @@ __global__ __launch_bounds__(256) void k_realspace_bal(RSArgs a) {
       }
    }
    const int np = __shfl(icnt, 63, 64), total = __shfl(itot, 63, 64);
+   // sites with up to 64 RSB_NBW candidates find a candidate's piece from a
+   // bit per piece start (the count of starts up to it); others by binary
+   // search over the piece starts
+   const bool usebits = total <= a.bits_cap;
+   if (usebits && lane < RSB_NBW) sbits[lane] = 0;
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
    {
       int k = icnt - cnt, st = itot - tot;
 #pragma unroll
       for (int q = 0; q < 8; q++)
          if (pn[q] > 0) {
-            sst[wave][k] = st;
-            spd[wave][k] = make_int2(pj[q] - st, pz[q]);
+            sst[k] = st;
+            spd[k] = make_int2(pj[q] - st, pz[q]);
+            if (usebits) atomicOr(&sbits[st >> 6], 1ull << (st & 63));
             k++;
             st += pn[q];
          }
-      if (lane == 63) sst[wave][np] = total;
+      if (lane == 63) sst[np] = total;
    }
    // the wave's own LDS writes before its reads
    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -318,19 +343,30 @@ __global__ __launch_bounds__(256) void k_realspace_bal(RSArgs a) {
    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
    R fx = 0, fy = 0, fz = 0, ep = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
-   int kf = 0;   // the piece holding the batch's first candidate
+   int kf = 0;   // the piece holding the batch's first candidate (binary search)
+   int nst = 0;  // piece starts before the batch (bits); the first piece starts at 0
    for (int base = 0; base < total; base += 64) {
       const int c = base + lane;
-      // last piece k in [kf, kf + 64) with sst[k] <= c (each piece holds a
-      // candidate, so the 64 candidates of the batch lie in that range)
-      const int kend = min(kf + 64, np);
-      int k = kf;
+      int k;
+      if (usebits) {
+         // piece = starts at or before c, less one; bits past total are
+         // clear, so k <= np - 1 on every lane
+         const unsigned long long w = sbits[base >> 6];
+         const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(w >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)w, 0));
+         k = nst + below + (int)((w >> lane) & 1ull) - 1;
+         nst += __popcll(w);
+      } else {
+         // last piece k in [kf, kf + 64) with sst[k] <= c (each piece holds
+         // a candidate, so the 64 candidates of the batch lie in that range)
+         const int kend = min(kf + 64, np);
+         k = kf;
 #pragma unroll
-      for (int s = 32; s >= 1; s >>= 1)
-         if (k + s < kend && sst[wave][k + s] <= c) k += s;
-      kf = __shfl(k, 63, 64);
+         for (int s = 32; s >= 1; s >>= 1)
+            if (k + s < kend && sst[k + s] <= c) k += s;
+         kf = __shfl(k, 63, 64);
+      }
       const bool act = c < total;
-      const int2 pd = spd[wave][k];
+      const int2 pd = spd[k];
       const int j = act ? c + pd.x : p;
       const int2 mj = a.cmeta[j];
       R xj, yj, zj, qj;
@@ -397,9 +433,10 @@ __global__ __launch_bounds__(256) void k_realspace_bal(RSArgs a) {
 
 template <class R, bool COUL>
 void launch_pt(int pt, bool bal, dim3 g, dim3 b, hipStream_t s, const RSArgs &a) {
+   const size_t shm = bal ? rsb_lds_bytes(2 * a.ncol, a.max_id * a.max_id * MGPU_NPOTP, sizeof(R)) : 0;
 #define RS_CASE(n)                                                           \
    case n:                                                                   \
-      if (bal) k_realspace_bal<R, n, COUL><<<g, b, 0, s>>>(a);               \
+      if (bal) k_realspace_bal<R, n, COUL><<<g, b, shm, s>>>(a);             \
       else k_realspace<R, n, COUL><<<g, b, 0, s>>>(a);                       \
       break;
    switch (pt) {
@@ -418,7 +455,7 @@ void mgpu::run_realspace(mgpu_engine *e) {
    // not split into columns, shell parts per site: at least 4, more (up to
    // 256, four waves of one 256-thread block) until the grid holds about 16
    // waves per CU
-   const bool bal = e->rs_ncol > 0 && (e->rs_mode == MGPU_RS_BALANCED ||
+   const bool bal = e->rs_ncol > 0 && (e->rs_mode == MGPU_RS_BALANCED || e->rs_mode == MGPU_RS_BALANCED_SEARCH ||
                                        (e->rs_mode == MGPU_RS_AUTO && (long)e->nsites >= 16L * e->ncu));
    e->rs_last_bal = bal;
    int lognq = 2;
@@ -456,6 +493,7 @@ void mgpu::run_realspace(mgpu_engine *e) {
    a.ccell = e->d_ccell.p;
    a.nsites = e->nsites;
    a.ncol = e->rs_ncol; a.own_col = e->rs_own_col;
+   a.bits_cap = e->rs_mode == MGPU_RS_BALANCED_SEARCH ? 0 : 64 * RSB_NBW;
    a.cols = e->d_rscol.p; a.colD = e->d_rscolD.p; a.ccz = e->d_ccz.p;
    a.cvx = (float)(e->h[2] / e->nz); a.cvy = (float)(e->h[5] / e->nz); a.cvz = (float)(e->h[8] / e->nz);
    a.potpar = e->d_potpar.p;

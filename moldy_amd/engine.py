@@ -211,11 +211,12 @@ class ForceEngine:
         """Real-space pass on a second stream beside the Ewald sum."""
         self._chk(lib().mgpu_set_overlap(self.h_, int(bool(on))))
 
-    RS_AUTO, RS_WALK, RS_BALANCED = 0, 1, 2
+    RS_AUTO, RS_WALK, RS_BALANCED, RS_BALANCED_SEARCH = 0, 1, 2, 3
 
     def set_realspace_mode(self, mode):
-        """Real-space kernel: RS_AUTO, RS_WALK (per-offset walk) or
-        RS_BALANCED (one wave per site over the shell's z-columns)."""
+        """Real-space kernel: RS_AUTO, RS_WALK (per-offset walk),
+        RS_BALANCED (one wave per site over the shell's z-columns) or
+        RS_BALANCED_SEARCH (the same, candidates placed by binary search)."""
         self._chk(lib().mgpu_set_realspace_mode(self.h_, int(mode)))
 
     def realspace_balanced(self):

@@ -115,11 +115,12 @@ def test_force_calc_fp64(eng_mod, case):
 BALANCED_CASES = tuple(c for c in CASES if c != "clay")
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 @pytest.mark.parametrize("case", list(CASES))
 def test_force_calc_fp64_both_kernels(eng_mod, case, mode):
-    """The per-offset walk (1) and the load-balanced kernel (2) each against
-    the oracle: same pair set, forces, energy, virial and close-pair count."""
+    """The per-offset walk (1) and the load-balanced kernel, candidates placed
+    by piece-start bits (2) or by binary search (3), each against the oracle:
+    same pair set, forces, energy, virial and close-pair count."""
     s = CASES[case]()
     o = _oracle(s)
     xyz = o.make_sites()
@@ -127,7 +128,7 @@ def test_force_calc_fp64_both_kernels(eng_mod, case, mode):
     e = eng_mod.ForceEngine(s)
     e.set_realspace_mode(mode)
     got = e.force_calc(s.h, xyz)
-    assert e.realspace_balanced() == (mode == 2 and case in BALANCED_CASES)
+    assert e.realspace_balanced() == (mode >= 2 and case in BALANCED_CASES)
     assert rel_err(got["force"], ref["force"]) < TOL64
     assert abs(got["pe"] - ref["pe"]) <= TOL64 * max(abs(ref["pe"]), 1e-300)
     assert rel_err(np.triu(got["stress"]), np.triu(ref["stress"])) < TOL64
@@ -140,18 +141,19 @@ def test_mixed_realspace_kernels_agree(eng_mod, case):
     """In FP32 the two real-space kernels differ only in summation order."""
     s = CASES[case]()
     out = []
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         e = eng_mod.ForceEngine(s, precision=1)
         e.set_realspace_mode(mode)
         e.upload_state()
         pe, dip, st = e.eval_forces_device()
-        assert e.realspace_balanced() == (mode == 2)
+        assert e.realspace_balanced() == (mode >= 2)
         out.append((pe.copy(), st.copy(), e.download()[3].copy()))
         e.close()
-    (pa, sa, fa), (pb, sb, fb) = out
-    assert rel_err(fb, fa) < 1e-6
-    assert rel_err(sb, sa) < 1e-6
-    assert np.max(np.abs(pb - pa)) <= 1e-6 * np.max(np.abs(pa))
+    pa, sa, fa = out[0]
+    for pb, sb, fb in out[1:]:
+        assert rel_err(fb, fa) < 1e-6
+        assert rel_err(sb, sa) < 1e-6
+        assert np.max(np.abs(pb - pa)) <= 1e-6 * np.max(np.abs(pa))
 
 
 @pytest.mark.parametrize("case", [c for c in CASES if not c.startswith("argon")])
