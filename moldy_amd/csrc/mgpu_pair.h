@@ -63,15 +63,27 @@ template <int N> __host__ __device__ inline Dual<N> operator*(Dual<N> a, double 
 template <int N> __host__ __device__ inline Dual<N> operator*(double b, Dual<N> a) { return a * b; }
 
 template <class R> struct Math;
+// root(rsq, r, r_r): r = sqrt(rsq) and r_r = 1/r.  FP64 and Dual take the
+// correctly rounded square root and divide (the reference's arithmetic);
+// FP32 takes the hardware reciprocal square root (v_rsq_f32, about 1 ulp)
+// and r = rsq / sqrt(rsq) from it.
 template <> struct Math<double> {
    static __host__ __device__ inline double sqrt_(double x) { return sqrt(x); }
    static __host__ __device__ inline double exp_(double x) { return exp(x); }
    static __host__ __device__ inline double rcp(double x) { return 1.0 / x; }
+   static __host__ __device__ inline void root(double x, double &r, double &r_r) {
+      r = sqrt(x);
+      r_r = 1.0 / r;
+   }
 };
 template <> struct Math<float> {
    static __device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
    static __device__ __forceinline__ float exp_(float x) { return __expf(x); }
    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+   static __device__ __forceinline__ void root(float x, float &r, float &r_r) {
+      r_r = __builtin_amdgcn_rsqf(x);
+      r = x * r_r;
+   }
 };
 template <int N> struct Math<Dual<N>> {
    static __host__ __device__ inline Dual<N> sqrt_(Dual<N> x) {
@@ -92,6 +104,10 @@ template <int N> struct Math<Dual<N>> {
       for (int k = 0; k < N; k++) x.d[k] *= g;
       return x;
    }
+   static __host__ __device__ inline void root(Dual<N> x, Dual<N> &r, Dual<N> &r_r) {
+      r = sqrt_(x);
+      r_r = rcp(r);
+   }
 };
 
 // One pair: energy u and forceij = -U'(r)/r, restating kernel.c:188-460.
@@ -101,11 +117,10 @@ __host__ __device__ __forceinline__ void pair_term(R rsq, R qq, R norm, R alpha,
    using M = Math<R>;
    R r, r_r, r2r, t = 0, et = 0;
    if (COUL) {
-      r = M::sqrt_(rsq);
+      M::root(rsq, r, r_r);
       R ar = alpha * r;
       t = M::rcp(R(1) + R(AS_P) * ar);
       et = qq * M::exp_(-(ar * ar));
-      r_r = M::rcp(r);
       R poly = t * (R(AS_A1) + t * (R(AS_A2) + t * (R(AS_A3) + t * (R(AS_A4) + t * R(AS_A5)))));
       t = poly * et * r_r;
       et = t + norm * et;
@@ -115,8 +130,7 @@ __host__ __device__ __forceinline__ void pair_term(R rsq, R qq, R norm, R alpha,
       r = 0;
       r_r = 0;
    } else {
-      r = M::sqrt_(rsq);
-      r_r = M::rcp(r);
+      M::root(rsq, r, r_r);
       r2r = r_r * r_r;
    }
    if (PT == 0) {  // Lennard-Jones
