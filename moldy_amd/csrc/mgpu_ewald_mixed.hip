@@ -41,7 +41,6 @@ typedef float float2v __attribute__((ext_vector_type(2)));
 constexpr double PI_D = 3.14159265358979323846;
 constexpr int KS = 32;        // sites per MFMA k step
 constexpr int FROWS = 128;    // complex rows per forward workgroup
-constexpr int BSITES = 128;   // sites per backward workgroup
 
 __device__ __forceinline__ void split2(double v, half_t &hi, half_t &lo) {
    float f = (float)v;
@@ -539,7 +538,7 @@ struct FArgs {
    const half_t *Wt;
    const unsigned *maxbits;
    double *force;           // [3][nsp]
-   int nblk, nwg;           // site blocks of BSITES; workgroups (see k_mix_force)
+   int nblk, nwg;           // site blocks (one workgroup's sites); workgroups (see k_mix_force)
    int direct;              // one workgroup per whole block, nothing to wait for: add into force
    float *fpart;            // [segments][3][nsp] otherwise
    int hstep;               // X row of exp(i a*.r): 1, or 0 when hmax = 0 (no h steps)
@@ -559,24 +558,37 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
    return make_float2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
 }
 
-// Four waves of 32 sites each (two m tiles of 16), 128 sites a workgroup and
-// two workgroups per CU (NB <= 3), so one workgroup's epilogue runs beside
-// the other's MFMAs.  The weight tiles (16 rows, 4 kinds x hi/lo, K2
-// columns) are double-buffered in LDS and staged through registers: tile
-// t + 2 is loaded while tile t is multiplied and written to LDS after it,
-// so its loads have a whole tile to land and one barrier per tile orders
-// the buffers.
-template <int NB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mix_force(FArgs a) {
+// Waves of 32 sites each (two m tiles of 16) in groups of four, 128 sites a
+// group.  With two groups (NB <= 3: one 512-thread workgroup per CU, two
+// waves per SIMD) the groups take turns: while one group's waves run the
+// MFMAs of a row tile, the other group's waves -- one on each SIMD beside
+// them -- run the epilogue of their last tile, and a barrier swaps the
+// roles, so the matrix cores are fed while the epilogues run.  With one
+// group (NB > 3, one wave per SIMD) a wave multiplies, then forms its
+// epilogue.  Both groups read the same weight tiles, double-buffered in LDS
+// and staged by the first group through registers: tile t + 1 is loaded at
+// the start of the first group's epilogue of tile t and written to the
+// other buffer at its end, after the last reads of that buffer.
+template <int NB, int GROUPS>
+__global__ __launch_bounds__(256 * GROUPS) __attribute__((amdgpu_waves_per_eu(GROUPS))) void k_mix_force(FArgs a) {
    constexpr int NCOL = 16 * NB, K2 = 2 * NCOL;
-   constexpr int RW = K2 + 8;          // LDS row stride (halves): 16-byte reads of 16 rows hit distinct banks
-   constexpr int CPR = K2 / 8;         // 16-byte chunks per weight row
-   constexpr int NST = K2 / 16;        // chunks per thread and tile (8 kinds x 16 rows x CPR over 256 threads)
-   __shared__ __attribute__((aligned(16))) half_t sW[2][8][16][RW];
+   constexpr int RW = K2 + 16;         // LDS row stride (halves): the fewest bank conflicts measured for the fragment reads
+   constexpr int NST = K2 / 16;        // chunks per staging thread and tile (8 kinds x 16 rows x K2 / 8 over 256 threads)
+   constexpr int PL = 16 * RW + 8;     // LDS plane (kind x hi/lo) stride: planes start 16 bytes apart in the banks
+   constexpr int BS = 128 * GROUPS;    // sites per workgroup
+   __shared__ __attribute__((aligned(16))) half_t sW[2][8 * PL];
    // per site of the block: exp(i a*.r) (next row of a column) and
    // exp(16 i b*.r) (start of the lane's next column), side by side
-   __shared__ float2 sMul[BSITES][2];
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, lc = lane & 15;
+   __shared__ float2 sMul[BS][2];
+   // each lane's column-start phases B (its 8 sites), read and written only
+   // when its row starts a column
+   __shared__ float2 sB[4 * GROUPS][2][4][64];
+   // the wave index as a scalar: the group branches below are wave-uniform,
+   // so every wave meets exactly its own group's barriers
+   const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, lc = lane & 15;
+   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int grp = wave >> 2, wq = wave & 3;
+   const bool stager = grp == 0;
    const int ntile_all = a.nbp / 16;
    const long long U = (long long)a.nblk * ntile_all;
    const int wg = blockIdx.x;
@@ -585,42 +597,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
    const float inv_pw = 1.f / (a.sZ * pow2_scale(a.maxbits[0], 8192.f));
    const float inv_qw = 1.f / (a.sZ * pow2_scale(a.maxbits[1], 8192.f));
    const float c2q = a.c2 * (inv_qw / inv_pw);
+   // weight staging (first group): thread (row st / 16, c0 = st % 16) moves
+   // the NST consecutive 16-byte chunks of half of one plane's row: plane
+   // c0 / 2, chunks (c0 % 2) NST + j
+   const int st = tid & 255, sreg = (st & 15) >> 1, srow = st >> 4, sc = (st & 1) * NST;
+   const int soff = (sreg * a.nbp + srow) * K2 + 8 * sc;
+   const int doff = sreg * PL + srow * RW + 8 * sc;
+   half8 wst[NST];
+   auto load_w = [&](int rt) {
+      const half_t *src = a.Wt + (size_t)rt * 16 * K2 + soff;
+#pragma unroll
+      for (int j = 0; j < NST; j++) wst[j] = *(const half8 *)(src + 8 * j);
+   };
+   auto store_w = [&](int buf) {
+      half_t *dst = &sW[buf][doff];
+#pragma unroll
+      for (int j = 0; j < NST; j++) *(half8 *)(dst + 8 * j) = wst[j];
+   };
+   // a barrier on LDS only (table loads stay in flight); no instruction is
+   // scheduled across it
+   auto barrier = [&]() {
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+   };
    while (u < uend) {
       const int blk = (int)(u / ntile_all);
       const int rt0 = (int)(u - (long long)blk * ntile_all);
       const int ntile = (int)std::min<long long>(ntile_all, rt0 + (uend - u));
       const int seg = wg - first_wg((long long)blk * ntile_all, U, a.nwg);
-      const int base = a.s0 + blk * BSITES;
+      const int base = a.s0 + blk * BS, wbase = base + 128 * grp + 32 * wq;
       u += ntile - rt0;
       __syncthreads();   // the previous segment's LDS reads are done
-      // weight staging: thread (row tid / 16, c0 = tid % 16) moves chunks
-      // q = c0 + 16 j of its row's 8 CPR chunks (kind x hi/lo = q / CPR)
-      int soff[NST], doff[NST];
-#pragma unroll
-      for (int j = 0; j < NST; j++) {
-         const int q = (tid & 15) + 16 * j, reg = q / CPR, c = q - reg * CPR, row = tid >> 4;
-         soff[j] = (reg * a.nbp + row) * K2 + 8 * c;
-         doff[j] = (reg * 16 + row) * RW + 8 * c;
-      }
-      half8 wst[NST];
-      auto load_w = [&](int rt) {
-         const half_t *src = a.Wt + (size_t)rt * 16 * K2;
-#pragma unroll
-         for (int j = 0; j < NST; j++) wst[j] = *(const half8 *)(src + soff[j]);
-      };
-      auto store_w = [&](int buf) {
-         half_t *dst = &sW[buf][0][0][0];
-#pragma unroll
-         for (int j = 0; j < NST; j++) *(half8 *)(dst + doff[j]) = wst[j];
-      };
-      load_w(rt0);
+      if (stager) load_w(rt0);
       // Z operand [site][Zr | Zi over l = 0 .. NCOL-1], held for the whole
       // segment.  Sites past s1 read the last valid row: their products land
       // only in their own output rows, which are never stored.
       half8 zh[2][NB], zl[2][NB];
 #pragma unroll
       for (int m = 0; m < 2; m++) {
-         const int i = base + 32 * wave + 16 * m + lc, ic = i < a.s1 ? i : a.s1 - 1;
+         const int i = wbase + 16 * m + lc, ic = i < a.s1 ? i : a.s1 - 1;
          const size_t o = (size_t)ic * K2 + 8 * g;
 #pragma unroll
          for (int ks = 0; ks < NB; ks++) {
@@ -633,7 +649,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int m = 0; m < 2; m++)
 #pragma unroll
          for (int j = 0; j < 4; j++) f[m][j][0] = f[m][j][1] = f[m][j][2] = 0.f;
-      if (tid < BSITES) {
+      if (tid < BS) {
          const int i = base + tid, ic = i < a.s1 ? i : a.s1 - 1;
          sMul[tid][0] = a.X[(size_t)a.hstep * a.nsp + ic];
          sMul[tid][1] = a.Y16[ic];
@@ -642,7 +658,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       // its column's start B = exp(i k b*.r) (k < 0 through the conjugate of
       // exp(i |k| b*.r)) from the tables, for its 8 sites (sites past s1 read
       // the last valid site; their results are never stored)
-      float2 A[2][4], B[2][4];
+      float2 A[2][4];
       {
          int2 hk0 = a.bphk[rt0 * 16 + lc];
          if (hk0.x < 0) hk0 = make_int2(0, 0);
@@ -652,51 +668,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-               const int i = base + 32 * wave + 16 * m + 4 * g + j, ic = i < a.s1 ? i : a.s1 - 1;
-               const float2 y = yrow[ic];
-               B[m][j] = make_float2(y.x, ysg * y.y);
-               A[m][j] = cmulf(xrow[ic], B[m][j]);
+               const int i = wbase + 16 * m + 4 * g + j, ic = i < a.s1 ? i : a.s1 - 1;
+               const float2 y = yrow[ic], b = make_float2(y.x, ysg * y.y);
+               sB[wave][m][j][lane] = b;
+               A[m][j] = cmulf(xrow[ic], b);
             }
       }
-      float4 kv = a.bpk[rt0 * 16 + lc];
-      store_w(0);
-      load_w(rt0 + 1 < ntile ? rt0 + 1 : rt0);
-      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-      for (int rt = rt0; rt < ntile; rt++) {
-         const int buf = (rt - rt0) & 1;
-         const half_t *W = &sW[buf][0][0][0];
-         floatx4 acc[2][4];
+      // row data of the wave's next epilogue tile (kv) and of the tile after
+      // it (hk_n, kv_n), loaded an epilogue ahead
+      int ep = rt0;
+      float4 kv = a.bpk[ep * 16 + lc];
+      int2 hk_n = a.bphk[(ep + 1 < ntile ? ep + 1 : ep) * 16 + lc];
+      float4 kv_n = a.bpk[(ep + 1 < ntile ? ep + 1 : ep) * 16 + lc];
+      if (stager) store_w(0);
+      barrier();
+      floatx4 acc[2][4];
+      // weight fragments (hi, lo) of pair p = (ks, kd) run two pairs ahead of
+      // their six MFMAs through a ring of three register pairs; scheduling
+      // fences keep each pair's reads where they are issued, so the MFMAs of
+      // pair p wait only for reads two pairs old
+      auto mfma = [&](int buf) {
+         const half_t *W = &sW[buf][0] + lc * RW + 8 * g;
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int kd = 0; kd < 4; kd++) acc[m][kd] = floatx4{0, 0, 0, 0};
+         half8 fb[3][2];
+         auto rd = [&](int p) {
+            const int ks = p >> 2, kd = p & 3;
+            fb[p % 3][0] = *(const half8 *)(W + (2 * kd) * PL + ks * 32);
+            fb[p % 3][1] = *(const half8 *)(W + (2 * kd + 1) * PL + ks * 32);
+         };
+         rd(0);
+         if (4 * NB > 1) rd(1);
+         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
          for (int p = 0; p < 4 * NB; p++) {
             const int ks = p >> 2, kd = p & 3;
-            const half8 bh = *(const half8 *)(W + ((2 * kd) * 16 + lc) * RW + ks * 32 + 8 * g);
-            const half8 bl = *(const half8 *)(W + ((2 * kd + 1) * 16 + lc) * RW + ks * 32 + 8 * g);
+            const half8 bh = fb[p % 3][0], bl = fb[p % 3][1];
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
             }
+            if (p + 2 < 4 * NB) rd(p + 2);
+            __builtin_amdgcn_sched_barrier(0);
          }
-         // issue order: the weight fragments of pair p + 1 (two LDS reads) go
-         // out before the six MFMAs of pair p, hiding the LDS latency
-         __builtin_amdgcn_sched_group_barrier(0x0100, 4, 0);
-#pragma unroll
-         for (int p = 0; p < 4 * NB; p++) {
-            __builtin_amdgcn_sched_group_barrier(0x0008, 6, 0);
-            if (p + 2 < 4 * NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
-         }
-         const int rnext = rt + 1 < ntile ? rt + 1 : rt;
-         const int2 hk_n = a.bphk[rnext * 16 + lc];
-         const float4 kv_n = a.bpk[rnext * 16 + lc];
-         // P and Q carry power-of-two scales: the sums run in units of the
-         // P scale (exact, so the same bits as scaling every term) and the Q
-         // terms join through c2 times the ratio of the two scales.  Padding
-         // rows have zero weights and zero k, so they add exact zeros.
+      };
+      // forces of the tile ep from acc, then the phase of the lane's next
+      // row: h + 1 in the same column, or the start (h = 0) of column k + 16.
+      // P and Q carry power-of-two scales: the sums run in units of the P
+      // scale (exact, so the same bits as scaling every term) and the Q terms
+      // join through c2 times the ratio of the two scales.  Padding rows
+      // have zero weights and zero k, so they add exact zeros.
+      auto epilogue = [&]() {
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
@@ -708,25 +734,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                f[m][j][1] += kv.y * t1;
                f[m][j][2] += kv.z * t1 + c2q * t2;
             }
-         // the phase of the lane's next row: h + 1 in the same column, or the
-         // start (h = 0) of the column k + 16
          const bool cs = hk_n.x == 0;
+         if (cs) {
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+               for (int j = 0; j < 4; j++) A[m][j] = sB[wave][m][j][lane];
+         }
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-               const float2 mu = sMul[32 * wave + 16 * m + 4 * g + j][cs];
-               const float2 na = cmulf(cs ? B[m][j] : A[m][j], mu);
-               A[m][j] = na;
-               B[m][j] = cs ? na : B[m][j];
-            }
-         // tile rt + 1 into the other buffer (last read in tile rt - 1, before
-         // the previous barrier), tile rt + 2 into the registers
-         store_w(buf ^ 1);
-         load_w(rt + 2 < ntile ? rt + 2 : ntile - 1);
-         asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            for (int j = 0; j < 4; j++) A[m][j] = cmulf(A[m][j], sMul[128 * grp + 32 * wq + 16 * m + 4 * g + j][cs]);
+         if (cs) {
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+               for (int j = 0; j < 4; j++) sB[wave][m][j][lane] = A[m][j];
+         }
+         ep++;
          kv = kv_n;
+         const int e2 = ep + 1 < ntile ? ep + 1 : ntile - 1;
+         hk_n = a.bphk[e2 * 16 + lc];
+         kv_n = a.bpk[e2 * 16 + lc];
+      };
+      for (int rt = rt0; rt < ntile; rt++) {
+         const int buf = (rt - rt0) & 1;
+         if (grp == 0) {
+            mfma(buf);
+            if (GROUPS == 2) barrier();
+            // tile rt + 1 into the other buffer (its last reads were the
+            // MFMAs of tile rt - 1, before the previous barrier); its loads
+            // (L2-resident weights) land during the epilogue
+            if (rt + 1 < ntile) load_w(rt + 1);
+            epilogue();
+            if (rt + 1 < ntile) store_w(buf ^ 1);
+            barrier();
+         } else {
+            if (rt > rt0) epilogue();
+            barrier();
+            mfma(buf);
+            barrier();
+         }
       }
+      if (grp == 1) epilogue();
       // lanes with the same g hold the same sites: sum over the 16 row lanes
 #pragma unroll
       for (int m = 0; m < 2; m++)
@@ -746,7 +796,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-               const int i = base + 32 * wave + 16 * m + 4 * g + j;
+               const int i = wbase + 16 * m + 4 * g + j;
                if (i < a.s1) {
                   if (a.direct) {
                      a.force[i] += (double)f[m][j][0];
@@ -765,11 +815,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 // Segment partials into the site forces, in segment order.
-__global__ void k_mix_fsum(int s0, int s1, int nsp, int ntile_all, long long U, int nwg,
+__global__ void k_mix_fsum(int s0, int s1, int nsp, int bs, int ntile_all, long long U, int nwg,
                            const float *__restrict__ fpart, double *__restrict__ force) {
    const int i = s0 + blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= s1) return;
-   const long long b0 = (long long)((i - s0) / BSITES) * ntile_all;
+   const long long b0 = (long long)((i - s0) / bs) * ntile_all;
    const int nseg = first_wg(b0 + ntile_all - 1, U, nwg) - first_wg(b0, U, nwg) + 1;
    for (int c = 0; c < 3; c++) {
       double t = 0.0;
@@ -782,8 +832,11 @@ __global__ void k_mix_fsum(int s0, int s1, int nsp, int ntile_all, long long U, 
 template <int NB> void launch_sk(dim3 g, hipStream_t s, const SkArgs &a) {
    k_mix_sk<NB><<<g, 512, 0, s>>>(a);
 }
+// two groups of four waves (ping-pong) where two waves fit on a SIMD
+constexpr int force_groups(int nb) { return nb <= 3 ? 2 : 1; }
 template <int NB> void launch_force(dim3 g, size_t, hipStream_t s, const FArgs &a) {
-   k_mix_force<NB><<<g, 256, 0, s>>>(a);
+   constexpr int G = force_groups(NB);
+   k_mix_force<NB, G><<<g, 256 * G, 0, s>>>(a);
 }
 
 #define NB_SWITCH(nb, CALL)                                                     \
@@ -897,9 +950,10 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
    // tiles a workgroup, each workgroup an equal share of the units
    struct FGrid { int nblk, nwg, maxseg; };
    const int ntile_bp = nbp / 16;
-   const int fslots = (NB <= 3 ? 2 : 1) * e->ncu;
+   const int fbs = 128 * force_groups(NB);   // sites per workgroup
+   const int fslots = e->ncu;                 // one resident workgroup per CU
    auto force_grid = [&](int s0, int s1) {
-      FGrid fg{(s1 - s0 + BSITES - 1) / BSITES, 0, 1};
+      FGrid fg{(s1 - s0 + fbs - 1) / fbs, 0, 1};
       if (fg.nblk <= 0) return fg;
       const long long U = (long long)fg.nblk * ntile_bp;
       fg.nwg = e->tune_force_parts <= 0
@@ -1007,7 +1061,7 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
          if (!f.direct) {
             join_side(e);
             k_mix_fsum<<<(s1 - s0 + 255) / 256, 256, 0, e->stream>>>(
-                s0, s1, nsp, ntile_bp, (long long)fg.nblk * ntile_bp, fg.nwg, fpart, e->d_force_site.p);
+                s0, s1, nsp, fbs, ntile_bp, (long long)fg.nblk * ntile_bp, fg.nwg, fpart, e->d_force_site.p);
          }
       };
       run(0, e->nsitesxf, Wt, fgn);

@@ -72,3 +72,18 @@ def two_species_generic(ptype, seed=21, n=150):
     sp = [system.Species("a", n, [1], np.zeros((1, 3))), system.Species("b", n, [2], np.zeros((1, 3)))]
     return system.MoldySystem(sp, np.array([0, 20.0, 30.0]), q, pp, ptype, ctl,
                               np.eye(3) * L, pos, np.zeros((0, 4)), f"pair{ptype}")
+
+
+def tall_water(nx=6, ny=6, nz=40, a=3.1, seed=14, cutoff=8.0, alpha=0.40):
+    """Water on a jittered nx x ny x nz lattice of spacing a (a box 20 times
+    taller than wide): the k set reaches l = 58, beyond 48 columns, which
+    takes the one-group MFMA back-projection."""
+    s = system.water_box(nx * ny * nz, seed=seed, cutoff=cutoff, alpha=alpha, name="tallwater")
+    rng = np.random.default_rng(seed)
+    grid = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    n = np.array([nx, ny, nz], float)
+    pos = (grid + 0.5) / n - 0.5 + rng.normal(0, 0.1 / (a * n), grid.shape)
+    s.c_of_m = pos - np.floor(pos + 0.5)
+    s.h = np.diag(a * n)
+    return s
+

@@ -58,3 +58,24 @@ def test_mixed_eval_vs_fp64(case):
     assert errs["stress"] < TOL_S
     e64.close()
     emx.close()
+
+
+def test_mixed_one_group_backprojection():
+    """A box 20 times taller than wide: the k set reaches l = 58, so the MFMA
+    back-projection runs with one group of waves (more than 48 columns); the
+    mixed evaluation holds the same bar against FP64 as everywhere else."""
+    from moldy_amd import engine as E
+    s = S.tall_water()
+    e64 = E.ForceEngine(s, precision=E.FP64)
+    r64 = e64.eval_forces()
+    emx = E.ForceEngine(s, precision=E.MIXED)
+    rmx = emx.eval_forces()
+    assert emx.last_info()[1].lmax >= 48
+    err = rel_err(rmx["site_force"], r64["site_force"])
+    print("tall water: site force rms rel", f"{err:.2e}")
+    assert err < TOL_F
+    assert rel_err(rmx["force"], r64["force"]) < TOL_F
+    assert abs(rmx["pe"][1] - r64["pe"][1]) / abs(r64["pe"][1]) < TOL_E
+    assert rel_err(rmx["stress"], r64["stress"]) < TOL_S
+    e64.close()
+    emx.close()
