@@ -36,8 +36,8 @@ PEAK = {"fp64_vector": 78.6e12, "fp64_mfma": 78.6e12, "fp32_vector": 157.3e12, "
 # passes on this workload, scaled by the counters' reading of a plain
 # 512 MiB copy (profiles/saltwater132872_eval_forces_hbm_traffic.md).
 MIB = float(1 << 20)
-TRAFFIC_BYTES = {"ewald_mfma_force": (298.66 + 6.08) * MIB, "ewald_mfma_sk": (653.42 + 72.36) * MIB,
-                 "ewald_sfac": (331.53 + 109.13) * MIB, "ewald_force": (1681.52 + 97.32) * MIB}
+TRAFFIC_BYTES = {"ewald_mfma_force": (302.33 + 6.25) * MIB, "ewald_mfma_sk": (661.18 + 102.74) * MIB,
+                 "ewald_sfac": (328.84 + 107.53) * MIB, "ewald_force": (1422.77 + 24.33) * MIB}
 
 
 def build_workload(small=False):
