@@ -757,26 +757,25 @@ __global__ __launch_bounds__(256 * GROUPS) __attribute__((amdgpu_waves_per_eu(GR
          hk_n = a.bphk[e2 * 16 + lc];
          kv_n = a.bpk[e2 * 16 + lc];
       };
+      // Both groups run the same loop body; the second starts one barrier
+      // later and the first ends with one barrier more, so phase 2t holds the
+      // first group's MFMAs of tile t beside the second group's epilogue of
+      // tile t - 1, and phase 2t + 1 the first group's epilogue (and the
+      // staging of tile t + 1) beside the second group's MFMAs of tile t.
+      if (GROUPS == 2 && grp == 1) barrier();
       for (int rt = rt0; rt < ntile; rt++) {
          const int buf = (rt - rt0) & 1;
-         if (grp == 0) {
-            mfma(buf);
-            if (GROUPS == 2) barrier();
-            // tile rt + 1 into the other buffer (its last reads were the
-            // MFMAs of tile rt - 1, before the previous barrier); its loads
-            // (L2-resident weights) land during the epilogue
-            if (rt + 1 < ntile) load_w(rt + 1);
-            epilogue();
-            if (rt + 1 < ntile) store_w(buf ^ 1);
-            barrier();
-         } else {
-            if (rt > rt0) epilogue();
-            barrier();
-            mfma(buf);
-            barrier();
-         }
+         mfma(buf);
+         if (GROUPS == 2) barrier();
+         // tile rt + 1 into the other buffer (its last reads were the MFMAs
+         // of tile rt - 1, before the previous barrier); its loads
+         // (L2-resident weights) land during the epilogue
+         if (stager && rt + 1 < ntile) load_w(rt + 1);
+         epilogue();
+         if (stager && rt + 1 < ntile) store_w(buf ^ 1);
+         barrier();
       }
-      if (grp == 1) epilogue();
+      if (GROUPS == 2 && grp == 0) barrier();
       // lanes with the same g hold the same sites: sum over the 16 row lanes
 #pragma unroll
       for (int m = 0; m < 2; m++)
