@@ -172,8 +172,11 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
    constexpr int NCOL = 16 * NB;
    constexpr int BCH = 16 * NCOL;              // 16-byte Z chunks per step
    constexpr int BPT = (BCH + 511) / 512;      // per thread
-   __shared__ __attribute__((aligned(16))) half_t sA[2][4][FROWS][KS + 8];       // Ar hi/lo, Ai hi/lo
-   __shared__ __attribute__((aligned(16))) half_t sB[2][2][2][NCOL][KS + 8];     // [buf][Zr|Zi][hi|lo][col][site]
+   // tile rows padded by 16 halves (8 where the LDS would not hold that):
+   // the fewest bank conflicts measured for the fragment reads
+   constexpr int SKP = NB <= 3 ? 16 : 8;
+   __shared__ __attribute__((aligned(16))) half_t sA[2][4][FROWS][KS + SKP];     // Ar hi/lo, Ai hi/lo
+   __shared__ __attribute__((aligned(16))) half_t sB[2][2][2][NCOL][KS + SKP];   // [buf][Zr|Zi][hi|lo][col][site]
    __shared__ int2 shk[FROWS];
    const int tile = blockIdx.x, slice = blockIdx.y;
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
