@@ -686,36 +686,37 @@ __global__ __launch_bounds__(256 * GROUPS) __attribute__((amdgpu_waves_per_eu(GR
       if (stager) store_w(0);
       barrier();
       floatx4 acc[2][4];
-      // weight fragments (hi, lo) of pair p = (ks, kd) run two pairs ahead of
-      // their six MFMAs through a ring of three register pairs; scheduling
+      // weight fragments (hi, lo) of pair p = (ks, kd) run FD pairs ahead of
+      // their six MFMAs through a ring of FD + 1 register pairs; scheduling
       // fences keep each pair's reads where they are issued, so the MFMAs of
-      // pair p wait only for reads two pairs old
+      // pair p wait only for reads FD pairs old
+      constexpr int FD = 3;
       auto mfma = [&](int buf) {
          const half_t *W = &sW[buf][0] + lc * RW + 8 * g;
 #pragma unroll
          for (int m = 0; m < 2; m++)
 #pragma unroll
             for (int kd = 0; kd < 4; kd++) acc[m][kd] = floatx4{0, 0, 0, 0};
-         half8 fb[3][2];
+         half8 fb[FD + 1][2];
          auto rd = [&](int p) {
             const int ks = p >> 2, kd = p & 3;
-            fb[p % 3][0] = *(const half8 *)(W + (2 * kd) * PL + ks * 32);
-            fb[p % 3][1] = *(const half8 *)(W + (2 * kd + 1) * PL + ks * 32);
+            fb[p % (FD + 1)][0] = *(const half8 *)(W + (2 * kd) * PL + ks * 32);
+            fb[p % (FD + 1)][1] = *(const half8 *)(W + (2 * kd + 1) * PL + ks * 32);
          };
-         rd(0);
-         if (4 * NB > 1) rd(1);
+#pragma unroll
+         for (int p = 0; p < FD && p < 4 * NB; p++) rd(p);
          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
          for (int p = 0; p < 4 * NB; p++) {
             const int ks = p >> 2, kd = p & 3;
-            const half8 bh = fb[p % 3][0], bl = fb[p % 3][1];
+            const half8 bh = fb[p % (FD + 1)][0], bl = fb[p % (FD + 1)][1];
 #pragma unroll
             for (int m = 0; m < 2; m++) {
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bh, acc[m][kd], 0, 0, 0);
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zh[m][ks], bl, acc[m][kd], 0, 0, 0);
                acc[m][kd] = __builtin_amdgcn_mfma_f32_16x16x32_f16(zl[m][ks], bh, acc[m][kd], 0, 0, 0);
             }
-            if (p + 2 < 4 * NB) rd(p + 2);
+            if (p + FD < 4 * NB) rd(p + FD);
             __builtin_amdgcn_sched_barrier(0);
          }
       };
