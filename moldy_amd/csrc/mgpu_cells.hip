@@ -306,11 +306,15 @@ struct PLArgs {
 
 // One wavefront per reference cell.  Lanes take offsets; each lane counts
 // the sites its neighbour cell contributes for the current frame type, a
-// wave prefix sum places them, and (fill mode) the lane writes them.  The
+// wave prefix sum places them, and (fill mode) they are written.  The
 // cell-ordered arrays hold each cell's sites contiguously in the reference's
 // list order (entries in linked-list order, a molecule's sites in order), so
 // a neighbour cell's contribution is its run of corig, filtered by the frame
-// flag when a framework is present.
+// flag when a framework is present.  Without a framework the lanes then take
+// the 64 offsets' output entries 64 at a time: a bit per run start marks
+// where each non-empty run begins, an entry's run is the count of starts up
+// to it (v_mbcnt), and every store is a whole wave's consecutive entries.
+constexpr int PL_NBW = 8;   // 64-bit start words per wave: up to 512 entries per 64 offsets
 template <bool FILL>
 static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *__restrict__ off,
                                                          const int *__restrict__ css,
@@ -320,9 +324,11 @@ static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *_
                                                          const int *__restrict__ lstart,
                                                          int *__restrict__ nab, int *__restrict__ pbct,
                                                          int *__restrict__ nfnab) {
-   int lane = threadIdx.x & 63;
-   int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-   if (c >= a.ncells) return;
+   __shared__ int2 srun[4][64];                       // (first site - first entry, image code) of each run
+   __shared__ unsigned long long sbit[4][PL_NBW];     // run starts
+   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   int c = blockIdx.x * 4 + w;
+   if (c >= a.ncells) return;   // wave-uniform; the LDS below is the wave's own
    int ix = c / (a.ny * a.nz), iy = c / a.nz - a.ny * ix, iz = c - a.nz * (iy + a.ny * ix);
    int base = FILL ? lstart[c] : 0;
    int nn = 0;
@@ -346,7 +352,42 @@ static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *_
             }
          }
          int inc = wave_incl_scan(cnt, lane);
-         if (FILL && cnt) {
+         const int tot = __shfl(inc, 63, 64);
+         if (FILL && a.nftypes == 1 && tot <= 64 * PL_NBW) {
+            const int st = inc - cnt;
+            if (lane < PL_NBW) sbit[w][lane] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const unsigned long long ne = __ballot(cnt > 0);
+            if (cnt > 0) {
+               const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(ne >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ne, 0));
+               srun[w][rank] = make_int2(j0 - st, code);
+               atomicOr(&sbit[w][st >> 6], 1ull << (st & 63));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // run of entry q = starts at or before q, less one (the first
+            // non-empty run starts at 0; bits past tot are clear)
+            int nst = 0;
+            for (int q0 = 0; q0 < tot; q0 += 64) {
+               const unsigned long long bw = sbit[w][q0 >> 6];
+               const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(bw >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bw, 0));
+               const int k = nst + below + (int)((bw >> lane) & 1ull) - 1;
+               nst += __popcll(bw);
+               const int q = q0 + lane;
+               if (q < tot) {
+                  const int2 rd = srun[w][k];
+                  nab[base + nn + q] = corig[q + rd.x];
+                  pbct[base + nn + q] = rd.y;
+               }
+            }
+            // the next offsets' writes to srun/sbit come after these reads
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+         } else if (FILL && cnt) {
             int p = base + nn + inc - cnt;
             for (int j = j0; j < j1; j++) {
                if (a.nftypes != 1 && (cmeta[j].x >> 8) != ft) continue;
@@ -355,7 +396,7 @@ static __global__ __launch_bounds__(256) void k_pairlist(PLArgs a, const int4 *_
                p++;
             }
          }
-         nn += __shfl(inc, 63, 64);
+         nn += tot;
       }
       if (FILL && lane == 0) nfnab[2 * c + ft] = nn;
    }
