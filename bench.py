@@ -65,7 +65,7 @@ def time_gpu(eng, steps, warmup):
     t1 = time.perf_counter()
     prof = {}
     for name in ["binning", "realspace", "ewald_trig", "ewald_sfac", "ewald_kspace",
-                 "ewald_force", "ewald_mfma_sk", "ewald_mfma_force", "ewald_mfma_prep",
+                 "ewald_force", "ewald_mfma_sk", "ewald_mfma_reduce", "ewald_mfma_force", "ewald_mfma_prep",
                  "make_sites", "glue"]:
         ms, n = eng.profile_get(name)
         if n:

@@ -450,15 +450,17 @@ __global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
       double *dn = a.Wnd + 2 * (size_t)idx;
       dn[0] = wnr; dn[1] = wni;
    }
-   // block maxima (positive floats compare as their bit patterns)
+   // block maxima (positive floats compare as their bit patterns), one
+   // atomic per block and quantity: atomics on one word serialise
    for (int o = 32; o > 0; o >>= 1) {
       mw = fmaxf(mw, __shfl_xor(mw, o, 64));
       mq = fmaxf(mq, __shfl_xor(mq, o, 64));
    }
+   __shared__ float wmax[4][2];
    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
    if (lane == 0) {
-      if (mw > 0) atomicMax(&a.maxbits[0], __float_as_uint(mw));
-      if (mq > 0) atomicMax(&a.maxbits[1], __float_as_uint(mq));
+      wmax[wv][0] = mw;
+      wmax[wv][1] = mq;
    }
 #pragma unroll
    for (int q = 0; q < 7; q++) {
@@ -469,6 +471,11 @@ __global__ __launch_bounds__(256) void k_mix_reduce(RedArgs a) {
    if (threadIdx.x < 7)
       a.bpart[8 * (size_t)blockIdx.x + threadIdx.x] =
           ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+   if (threadIdx.x < 2) {
+      const float m = fmaxf(fmaxf(wmax[0][threadIdx.x], wmax[1][threadIdx.x]),
+                            fmaxf(wmax[2][threadIdx.x], wmax[3][threadIdx.x]));
+      if (m > 0) atomicMax(&a.maxbits[threadIdx.x], __float_as_uint(m));
+   }
 }
 
 __device__ __forceinline__ float pow2_scale(unsigned bits, float target) {
