@@ -72,7 +72,6 @@ struct TabArgs {
 // parts -- assembled in LDS and written as contiguous rows.  One wave of 64
 // sites per block.
 constexpr int ZT_SITES = 64;
-constexpr int ZT_MAXK2 = 192;   // 2 * 16 * NB for NB <= 6
 
 __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__restrict__ site,
                                                    const double *__restrict__ chg, float2 *__restrict__ X,
@@ -81,11 +80,12 @@ __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__re
                                                    half_t *__restrict__ Zlr, half_t *__restrict__ Zhi,
                                                    half_t *__restrict__ Zli, half_t *__restrict__ ZTh,
                                                    half_t *__restrict__ ZTl) {
-   // rows padded by 8 halves: 4-way instead of 16-way bank conflicts
-   __shared__ __attribute__((aligned(16))) half_t sh[2][ZT_SITES * (ZT_MAXK2 + 8)];
+   // rows padded by 8 halves: 4-way instead of 16-way bank conflicts; the
+   // LDS holds 2 x 64 rows of K2 + 8 halves (sized at launch)
+   extern __shared__ __attribute__((aligned(16))) half_t sh[];
    const int ncol = a.ncol, K2 = 2 * ncol, RS = K2 + 8;
    const int si = threadIdx.x, base = blockIdx.x * ZT_SITES, i = base + si;   // nsp is a multiple of 64
-   half_t *rh = &sh[0][si * RS], *rl = &sh[1][si * RS];
+   half_t *rh = &sh[si * RS], *rl = &sh[(ZT_SITES + si) * RS];
    for (int c = 0; c < K2; c++) {
       const int cc = c < ncol ? c : c - ncol;
       if (i >= a.ns || cc > a.lmax) rh[c] = rl[c] = (half_t)0;
@@ -145,8 +145,8 @@ __global__ __launch_bounds__(64) void k_mix_tables(TabArgs a, const double *__re
    const size_t ob = (size_t)base * K2;
    for (int t = si; t < nchunk; t += ZT_SITES) {
       const int row = t / cpr, c = t - row * cpr;
-      *(half8 *)(ZTh + ob + 8 * (size_t)t) = *(const half8 *)&sh[0][row * RS + 8 * c];
-      *(half8 *)(ZTl + ob + 8 * (size_t)t) = *(const half8 *)&sh[1][row * RS + 8 * c];
+      *(half8 *)(ZTh + ob + 8 * (size_t)t) = *(const half8 *)&sh[row * RS + 8 * c];
+      *(half8 *)(ZTl + ob + 8 * (size_t)t) = *(const half8 *)&sh[(ZT_SITES + row) * RS + 8 * c];
    }
 }
 
@@ -1016,8 +1016,9 @@ void mgpu::run_ewald_mixed(mgpu_engine *e) {
       TabArgs t;
       for (int i = 0; i < 9; i++) t.ks[i] = 2 * PI_D * e->hinv[i];
       t.hmax = e->hmax; t.kmax = e->kmax; t.lmax = lmax; t.ns = ns; t.nsp = nsp; t.ncol = NCOL; t.sZ = sZ;
-      k_mix_tables<<<nsp / ZT_SITES, ZT_SITES, 0, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Y16, Zhr,
-                                                               Zlr, Zhi, Zli, ZTh, ZTl);
+      const size_t tshm = sizeof(half_t) * 2 * ZT_SITES * (K2 + 8);
+      k_mix_tables<<<nsp / ZT_SITES, ZT_SITES, tshm, e->stream>>>(t, e->d_site.p, e->d_chg_site.p, X, Y, Y16,
+                                                                  Zhr, Zlr, Zhi, Zli, ZTh, ZTl);
    }
    {
       ProfScope ps(e, "ewald_mfma_sk");
