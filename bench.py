@@ -36,8 +36,8 @@ PEAK = {"fp64_vector": 78.6e12, "fp64_mfma": 78.6e12, "fp32_vector": 157.3e12, "
 # passes on this workload, scaled by the counters' reading of a plain
 # 512 MiB copy (profiles/saltwater132872_eval_forces_hbm_traffic.md).
 MIB = float(1 << 20)
-TRAFFIC_BYTES = {"ewald_mfma_force": (302.33 + 6.25) * MIB, "ewald_mfma_sk": (661.18 + 102.74) * MIB,
-                 "ewald_sfac": (328.84 + 107.53) * MIB, "ewald_force": (1422.77 + 24.33) * MIB}
+TRAFFIC_BYTES = {"ewald_mfma_force": (273.95 + 3.00) * MIB, "ewald_mfma_sk": (649.87 + 48.02) * MIB,
+                 "ewald_sfac": (329.91 + 107.64) * MIB, "ewald_force": (1435.49 + 24.33) * MIB}
 
 
 def build_workload(small=False):
