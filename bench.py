@@ -36,6 +36,12 @@ PEAK = {"fp64_vector": 78.6e12, "fp64_mfma": 78.6e12, "fp32_vector": 157.3e12, "
 # passes on this workload, scaled by the counters' reading of a plain
 # 512 MiB copy (profiles/saltwater132872_eval_forces_hbm_traffic.md).
 MIB = float(1 << 20)
+# MFMA flops issued per launch of the mixed back-projection on this workload:
+# 520 blocks of 256 sites x 8 waves x 140 row tiles (2,240 back-projection
+# rows) x 72 v_mfma_f32_16x16x32_f16 (4 kinds x 3 column blocks x 2 site
+# tiles x 3 hi/lo products) x 16384 flops; SQ_INSTS_MFMA reads the same
+# 41.9 million instructions per dispatch.
+ISSUED_FLOPS = {"ewald_mfma_force": 520 * 8 * 140 * 72 * 16384.0}
 TRAFFIC_BYTES = {"ewald_mfma_force": (273.95 + 3.00) * MIB, "ewald_mfma_sk": (649.87 + 48.02) * MIB,
                  "ewald_sfac": (329.91 + 107.64) * MIB, "ewald_force": (1435.49 + 24.33) * MIB}
 
@@ -72,6 +78,18 @@ def time_gpu(eng, steps, warmup):
             prof[name] = ms
     eng.profile(False)
     return (t1 - t0) / steps, prof
+
+
+def mfma_rate(shape):
+    """Sustained TFLOP/s of one FP16 MFMA shape on this device (16:
+    16x16x32, 32: 32x32x16), from the library's probe."""
+    import ctypes
+    from moldy_amd import engine as E
+    v = ctypes.c_double()
+    rc = E.lib().mgpu_mfma_rate(0, shape, ctypes.byref(v))
+    if rc:
+        raise RuntimeError(f"mgpu_mfma_rate({shape}) failed: {rc}")
+    return v.value
 
 
 def pairlist_leg(eng, s, reps=5):
@@ -315,6 +333,15 @@ def main():
                                achieved=ach / 1e12, peak=peak / 1e12, unit="TFLOP/s",
                                frac=ach / peak, traffic=TRAFFIC_BYTES.get(dom),
                                flops_per_launch=kind[1], avg_ms=prof[dom])
+        if dom in ISSUED_FLOPS:
+            # the matrix-core work the kernel issues, beside the rate its
+            # instruction sustains on this device (the published dense peak
+            # assumes the faster 32x32x16 shape)
+            issued = ISSUED_FLOPS[dom] / (prof[dom] * 1e-3)
+            r16, r32 = mfma_rate(16), mfma_rate(32)
+            res["roofline"].update(instruction="v_mfma_f32_16x16x32_f16", issued=issued / 1e12,
+                                   instruction_peak=r16, issued_frac_of_instruction_peak=issued / 1e12 / r16,
+                                   mfma_32x32x16_peak=r32)
     res["kernel_ms"] = prof
     res["pairlist"] = pairlist_leg(eng, s)
     print(f"[bench] pairlist: {res['pairlist']}", file=sys.stderr, flush=True)

@@ -248,6 +248,12 @@ int mgpu_sync(mgpu_engine *eng);
 int mgpu_profile_enable(mgpu_engine *eng, int on);
 int mgpu_profile_reset(mgpu_engine *eng);
 int mgpu_profile_get(const mgpu_engine *eng, const char *name, double *avg_ms, long *calls);
+/* Sustained rate (TFLOP/s) of one FP16 matrix-core instruction on the
+ * device: shape 16 = v_mfma_f32_16x16x32_f16 (the shape of the mixed Ewald
+ * kernels), 32 = v_mfma_f32_32x32x16_f16; independent accumulator chains,
+ * two, four and eight waves per SIMD, the best of them.  A measurement aid with no
+ * counterpart in the reference. */
+int mgpu_mfma_rate(int device, int shape, double *tflops);
 
 #ifdef __cplusplus
 }

@@ -24,12 +24,14 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Wno-unused-variable", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INC, "-I" + CSRC]
-# the FP64 S(k) GEMM keeps its accumulators in VGPRs: in the AGPR form the
-# compiler copies all of them between the two register files every step
-FILE_FLAGS = {"mgpu_ewald.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# the FP64 S(k) GEMM and the matrix-core probe keep their accumulators in
+# VGPRs: in the AGPR form the compiler copies all of them between the two
+# register files every step
+FILE_FLAGS = {"mgpu_ewald.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              "mgpu_probe.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 SOURCES = ["mgpu_engine.hip", "mgpu_cells.hip", "mgpu_realspace.hip", "mgpu_ewald.hip", "mgpu_dynamics.hip",
            "mgpu_ewald_mixed.hip", "mgpu_glue.hip", "mgpu_adjoint.hip",
-           "mgpu_batch.hip"]
+           "mgpu_batch.hip", "mgpu_probe.hip"]
 
 
 def _run(cmd):
