@@ -97,6 +97,7 @@ static __global__ __launch_bounds__(256) void k_rc(const double *__restrict__ in
                                                   unsigned *__restrict__ count, double *__restrict__ tmp,
                                                   double *__restrict__ out) {
    __shared__ double ws[4][16];
+   __shared__ double fin[256];
    __shared__ bool last;
    const int r0 = blockIdx.x * chunk, r1 = min(nrows, r0 + chunk);
    double acc[16];
@@ -123,10 +124,18 @@ static __global__ __launch_bounds__(256) void k_rc(const double *__restrict__ in
    __syncthreads();
    if (last) {
       __threadfence();
+      // per = 256 / ncols threads a column, thread q of them summing the
+      // blocks q, q + per, ... in order; then the per sums in order
+      const int per = 256 / ncols, c = (int)threadIdx.x % ncols, q = (int)threadIdx.x / ncols;
+      double t = 0.0;
+      if (q < per)
+         for (unsigned b = q; b < gridDim.x; b += per) t += tmp[(size_t)b * ncols + c];
+      fin[threadIdx.x] = t;
+      __syncthreads();
       if ((int)threadIdx.x < ncols) {
-         double t = 0.0;
-         for (unsigned b = 0; b < gridDim.x; b++) t += tmp[(size_t)b * ncols + threadIdx.x];
-         out[threadIdx.x] = t;
+         double u = 0.0;
+         for (int k = 0; k < per; k++) u += fin[k * ncols + threadIdx.x];
+         out[threadIdx.x] = u;
       }
       if (threadIdx.x == 0) *count = 0u;
    }
@@ -134,7 +143,7 @@ static __global__ __launch_bounds__(256) void k_rc(const double *__restrict__ in
 
 void mgpu::reduce_cols(mgpu_engine *e, const double *in, int nrows, int ncols, double *out_dev) {
    if (ncols > 16) throw StatusError{MGPU_E_ARG, "reduce_cols: more than 16 columns"};
-   int nb = std::max(1, std::min(512, (nrows + 2047) / 2048));
+   int nb = std::max(1, std::min(1024, (nrows + 255) / 256));
    int chunk = (nrows + nb - 1) / nb;
    // d_rtmp: a completion count in its first slot, then the partials
    const double *before = e->d_rtmp.p;
