@@ -249,32 +249,35 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
    };
    // Z tile staging: chunk q = (part, hi/lo, column l, 8 sites c).  Padding
    // columns (l > lmax) read row lmax: the reduction never reads them.
-   half8 bst[BPT];
-   auto load_b = [&](int i0) {
+   // two register sets, like the X/Y rows: the set loaded a step ago goes
+   // to LDS while the other takes the tile two steps ahead
+   half8 bst[2][BPT];
+   auto load_b = [&](int i0, int set) {
 #pragma unroll
       for (int j = 0; j < BPT; j++) {
          const int q = tid + 512 * j < BCH ? tid + 512 * j : BCH - 1;
          const int c = q & 3, col = (q >> 2) % NCOL, hl = (q / (4 * NCOL)) & 1, part = q / (8 * NCOL);
          const half_t *tab = part ? (hl ? a.Zli : a.Zhi) : (hl ? a.Zlr : a.Zhr);
-         bst[j] = *(const half8 *)(tab + (size_t)(col < a.lmax ? col : a.lmax) * a.nsp + i0 + 8 * c);
+         bst[set][j] = *(const half8 *)(tab + (size_t)(col < a.lmax ? col : a.lmax) * a.nsp + i0 + 8 * c);
       }
    };
-   auto store_b = [&](int buf) {
+   auto store_b = [&](int buf, int set) {
 #pragma unroll
       for (int j = 0; j < BPT; j++) {
          const int q = tid + 512 * j;
          if (q < BCH) {
             const int c = q & 3, col = (q >> 2) % NCOL, hl = (q / (4 * NCOL)) & 1, part = q / (8 * NCOL);
-            *(half8 *)&sB[buf][part][hl][col][8 * c] = bst[j];
+            *(half8 *)&sB[buf][part][hl][col][8 * c] = bst[set][j];
          }
       }
    };
    int i0 = sl.x;
    if (i0 < sl.y) {
-      load_b(i0);
+      load_b(i0, 0);
       load_xy(i0, 0);
       gen_a(i0, 0, 0);
-      store_b(0);
+      store_b(0, 0);
+      load_b(i0 + KS < sl.y ? i0 + KS : i0, 1);
       load_xy(i0 + KS < sl.y ? i0 + KS : i0, 1);
    }
    __syncthreads();
@@ -291,9 +294,8 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
          ah[m] = *(const half8 *)&sA[buf][2 * wm][64 * rh + m * 16 + lc][8 * g];
          al8[m] = *(const half8 *)&sA[buf][2 * wm + 1][64 * rh + m * 16 + lc][8 * g];
       }
-      // table loads first: the next step's Z tile, then the X/Y rows two
-      // steps ahead
-      load_b(inext);
+      // table loads first: the Z tile and the X/Y rows two steps ahead
+      load_b(inext2, buf);
       load_xy(inext2, buf);
       // loads stay above; the A generation (VALU, LDS writes) may interleave
       // with the MFMAs below
@@ -318,9 +320,10 @@ __global__ __launch_bounds__(512) void k_mix_sk(SkArgs a) {
          __builtin_amdgcn_sched_group_barrier(0x0008, 12, 0);
          if (n + 2 < NB) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
       }
-      // the Z tile goes to LDS after the MFMAs, its loads a step old
+      // the next step's Z tile goes to LDS after the MFMAs, its loads a
+      // step old
       __builtin_amdgcn_sched_barrier(0);
-      store_b(buf ^ 1);
+      store_b(buf ^ 1, buf ^ 1);
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
    };
    while (i0 < sl.y) {
