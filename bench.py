@@ -42,8 +42,8 @@ MIB = float(1 << 20)
 # tiles x 3 hi/lo products) x 16384 flops; SQ_INSTS_MFMA reads the same
 # 41.9 million instructions per dispatch.
 ISSUED_FLOPS = {"ewald_mfma_force": 520 * 8 * 140 * 72 * 16384.0}
-TRAFFIC_BYTES = {"ewald_mfma_force": (263.69 + 2.25) * MIB, "ewald_mfma_sk": (652.92 + 72.09) * MIB,
-                 "ewald_sfac": (329.91 + 107.64) * MIB, "ewald_force": (1435.49 + 24.33) * MIB}
+TRAFFIC_BYTES = {"ewald_mfma_force": (271.76 + 2.25) * MIB, "ewald_mfma_sk": (653.42 + 47.88) * MIB,
+                 "ewald_sfac": (329.21 + 107.55) * MIB, "ewald_force": (1284.63 + 24.33) * MIB}
 
 
 def build_workload(small=False):
