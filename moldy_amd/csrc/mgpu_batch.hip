@@ -185,11 +185,12 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       HIPCHK(hipEventCreate(&t1));
       const bool prof = e->prof_on;
       e->prof_on = false;
+      int nrep = reps;
       auto time_eval = [&]() {
          e->geom_version++;   // re-plan with the new shape
          eval_forces_device(e, true);
          HIPCHK(hipEventRecord(t0, e->stream));
-         for (int k = 0; k < reps; k++) eval_forces_device(e, true);
+         for (int k = 0; k < nrep; k++) eval_forces_device(e, true);
          HIPCHK(hipEventRecord(t1, e->stream));
          HIPCHK(hipEventSynchronize(t1));
          float ms = 0;
@@ -200,9 +201,9 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       // whole rounds of the resident slots
       // two passes over the candidates, each scored by its faster pass, so
       // that a clock ramp or a neighbour's burst does not pick the shape
-      auto tune = [&](int &field, const std::vector<int> &cand) {
+      auto tune = [&](int &field, const std::vector<int> &cand, int passes = 2) {
          std::vector<float> t(cand.size(), 1e30f);
-         for (int pass = 0; pass < 2; pass++)
+         for (int pass = 0; pass < passes; pass++)
             for (size_t c = 0; c < cand.size(); c++) {
                field = cand[c];
                t[c] = std::min(t[c], time_eval());
@@ -216,7 +217,10 @@ void mgpu::autotune_engine(mgpu_engine *e, int reps, int *sk_blocks, int *npart)
       } else {
          tune(e->tune_npart, {4, 8, 16, 32});
       }
-      tune(e->overlap, {1, 0});   // real space beside the Ewald sum, or after it
+      // real space beside the Ewald sum, or after it: a difference of a few
+      // per cent, so longer trials and more passes than the shapes above
+      nrep = std::max(reps, 10);
+      tune(e->overlap, {1, 0}, 4);
       e->prof_on = prof;
       (void)hipEventDestroy(t0);
       (void)hipEventDestroy(t1);
