@@ -656,9 +656,11 @@ unsigned long mgpu::g_alloc_epoch = 0;
 mgpu::ProfScope::ProfScope(mgpu_engine *e_, const char *n) : e(e_), name(n) {
    if (!e->prof_on || e->capturing) return;
    if (e->ev_used * 2 + 2 > (int)e->ev_pool.size()) {
+      // timestamps only: no system-scope release at each record (the
+      // stream synchronisation before they are read orders everything)
       for (int k = 0; k < 64; k++) {
          hipEvent_t ev;
-         HIPCHK(hipEventCreate(&ev));
+         HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
          e->ev_pool.push_back(ev);
       }
    }
